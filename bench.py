@@ -1,0 +1,147 @@
+"""Benchmark of the fdtdz() hot path on one MI355X.
+
+One bench "step" is one full fdtdz() simulation of the workload (every FDTD
+time step of it), inputs already resident in HBM.  Default workload:
+native_512 -- 512x512x128 cells, 10000 FDTD steps, 8 snapshots, fp32.
+
+Prints ONE JSON line: Gcell-updates/s over the timed simulations, the HBM
+roofline of the stepper kernel (algorithmic bytes per launch / measured
+average launch duration, HIP events on the kernel's stream), and the FP64
+CPU oracle timed on a bounded sample of the same workload on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                       [--kernel auto|reference|stream] [--no-cpu-baseline]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+  sys.path.insert(0, ROOT)
+
+METRIC = ("Gcell-updates/s + % HBM roofline; batched fwd+grad vs plain at "
+          "≤1e-4 rel err")
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+
+
+def algorithmic_bytes_per_cell(zz, pml_lo, pml_hi):
+  """fp32 compulsory HBM bytes per cell per pass of one time step.
+
+  read E, H (24 B) + read cb (12 B) + write E, H (24 B) = 60 B; CPML psi
+  (4 components, read + write) 32 B on the pml_lo + pml_hi PML cells of each
+  z column; absorber mask 12 B per (x, y) column.
+  """
+  return 60.0 + 32.0 * (pml_lo + pml_hi) / zz + 12.0 / zz
+
+
+def cpu_baseline(kw, target_s=10.0):
+  """Times the FP64 C oracle (OpenMP) on a bounded sample of the workload."""
+  from oracle import c_oracle
+  from pjz_amd import workloads
+  try:
+    threads = len(os.sched_getaffinity(0))
+  except AttributeError:
+    threads = os.cpu_count() or 1
+  threads = max(1, min(threads, 16))
+  cells = workloads.cells(kw)
+  probe = 3
+  t0 = time.perf_counter()
+  c_oracle.fdtdz(**kw, threads=threads, nsteps=probe)
+  t_probe = time.perf_counter() - t0
+  n = int(max(probe, min(400, probe * target_s / max(t_probe, 1e-3))))
+  t0 = time.perf_counter()
+  c_oracle.fdtdz(**kw, threads=threads, nsteps=n)
+  wall = time.perf_counter() - t0
+  return {"value": round(cells * n / wall / 1e9, 5), "unit": "Gcell-updates/s",
+          "cores": threads, "kind": "port",
+          "sample": f"FP64 C oracle (oracle/fdtd_ref.c, OpenMP) on the full "
+                    f"{kw['epsilon'].shape[1]}x{kw['epsilon'].shape[2]}x"
+                    f"{kw['epsilon'].shape[3]} grid for {n} steps incl. set-up, "
+                    f"{wall:.1f} s wall"}
+
+
+def main():
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=3)
+  ap.add_argument("--warmup", type=int, default=1)
+  ap.add_argument("--workload", default="native_512")
+  ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "stream"])
+  ap.add_argument("--time-steps", type=int, default=0,
+                  help="override the FDTD step count (profiling runs only)")
+  ap.add_argument("--no-cpu-baseline", action="store_true")
+  args = ap.parse_args()
+  if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+    print("bench.py measures one GPU: the simulation is a single-GPU kernel and "
+          "multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+    sys.exit(2)
+
+  import numpy as np
+  import torch
+  from pjz_amd import fdtdz, workloads
+
+  builder = getattr(workloads, args.workload)
+  kw = builder(steps=args.time_steps) if args.time_steps else builder()
+  dev = torch.device("cuda", 0)
+  for k, v in list(kw.items()):
+    if isinstance(v, np.ndarray):
+      kw[k] = torch.as_tensor(v).to(dev)
+  kw["launch_params"] = dict(kernel=args.kernel)
+  cells = workloads.cells(kw)
+  tsteps = workloads.time_steps(kw)
+  lo, hi = kw["pml_widths"]
+  zz = kw["pml_sigma"].shape[0]
+
+  for _ in range(args.warmup):
+    fdtdz(**kw)
+  torch.cuda.synchronize()
+  stream = torch.cuda.current_stream(dev)
+  starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+  ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+  t0 = time.perf_counter()
+  for i in range(args.steps):
+    starts[i].record(stream)
+    fdtdz(**kw)
+    ends[i].record(stream)
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  gpu_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+  ms_per_step = wall * 1e3 / args.steps
+  value = cells * tsteps * args.steps / wall / 1e9
+
+  kernel_name = "reference" if args.kernel == "auto" else args.kernel
+  launches_per_step = 1  # stepper launches per FDTD step (reference: H+E pair)
+  bpc = algorithmic_bytes_per_cell(zz, lo, hi)
+  bytes_per_launch = cells * bpc
+  avg_launch_s = (sum(gpu_ms) / len(gpu_ms)) / 1e3 / (tsteps * launches_per_step)
+  achieved = bytes_per_launch / avg_launch_s / 1e9
+
+  line = {
+      "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
+      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+      "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+      "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+      "config": {"workload": f"{args.workload}: {kw['epsilon'].shape[1]}x"
+                             f"{kw['epsilon'].shape[2]}x{kw['epsilon'].shape[3]} cells, "
+                             f"{tsteps} FDTD steps, "
+                             f"{len(range(*kw['output_steps']))} snapshots",
+                 "cells": int(cells), "time_steps": int(tsteps), "kernel": kernel_name,
+                 "temporal_block": 1, "pml_widths": [int(lo), int(hi)]},
+      "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                   "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                   "traffic": None,
+                   "algorithmic_bytes_per_cell": round(bpc, 4),
+                   "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+  }
+  if not args.no_cpu_baseline:
+    np_kw = builder(steps=args.time_steps) if args.time_steps else builder()
+    line["cpu_baseline"] = cpu_baseline(np_kw)
+  print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+  main()

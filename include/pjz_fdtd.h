@@ -1,0 +1,108 @@
+/*
+ * pjz_fdtd.h -- C ABI of the MI355X (gfx950) Yee FDTD kernel library
+ * (libpjzfdtd.so).
+ *
+ * The library replaces the kernel call pjz makes at
+ *   src/pjz/_field.py:254-269   fields = fdtdz_jax.fdtdz(epsilon=..., dt=...,
+ *       source_field=..., source_waveform=..., source_position=...,
+ *       absorption_mask=..., pml_kappa=..., pml_sigma=..., pml_alpha=...,
+ *       pml_widths=..., output_steps=..., use_reduced_precision=...,
+ *       launch_params=..., offset=...)
+ * whose implementation is the un-vendored PyPI package fdtdz>=1.1.3
+ * (setup.py:26).  Every keyword of that call maps onto a field of
+ * pjz_fdtd_problem (static, hashable arguments) or onto a device pointer
+ * argument (traced arrays).
+ *
+ * Conventions
+ *   - Every array pointer is a DEVICE pointer to contiguous float32, C order,
+ *     z fastest, exactly the layouts pjz passes (shapes below).
+ *   - The caller owns all memory.  Scratch (field state, coefficients) comes
+ *     in as `workspace`, sized by pjz_fdtd_workspace_bytes().  The library
+ *     does not allocate device memory.
+ *   - Work is enqueued on `stream` (a hipStream_t, NULL = default stream) and
+ *     is stream-ordered; no host synchronisation happens inside the calls.
+ *   - Return 0 on success.  Non-zero codes: PJZ_FDTD_EINVAL (bad argument,
+ *     message names it), PJZ_FDTD_EHIP (HIP runtime error), PJZ_FDTD_ESPACE
+ *     (workspace too small).  pjz_fdtd_last_error() returns the calling
+ *     thread's last message.
+ *   - Calls are re-entrant; concurrent calls must use distinct workspaces.
+ */
+#ifndef PJZ_FDTD_H_
+#define PJZ_FDTD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PJZ_FDTD_ABI_VERSION 1
+
+#if defined(__GNUC__)
+#define PJZ_FDTD_API __attribute__((visibility("default")))
+#else
+#define PJZ_FDTD_API
+#endif
+
+#define PJZ_FDTD_OK 0
+#define PJZ_FDTD_EINVAL 1
+#define PJZ_FDTD_EHIP 2
+#define PJZ_FDTD_ESPACE 3
+
+/* Kernel selection (the `launch_params` analogue, field `kernel`). */
+#define PJZ_FDTD_KERNEL_AUTO 0
+#define PJZ_FDTD_KERNEL_REFERENCE 1 /* two launches per step, one thread/cell */
+#define PJZ_FDTD_KERNEL_STREAM 2    /* one fused launch per step, x-sweep    */
+
+/* Problem description: the static arguments of fdtdz(...). */
+typedef struct pjz_fdtd_problem {
+  int32_t xx, yy, zz;    /* domain: absorption_mask is (3,xx,yy); pml_* (zz,2)  */
+  int32_t ex, ey, ez;    /* epsilon is (3,ex,ey,ez), placed at (x0,y0,z0)       */
+  int32_t x0, y0, z0;    /* `offset`                                             */
+  int32_t pml_lo, pml_hi;/* `pml_widths`                                         */
+  int32_t src_axis;      /* 0/1/2: source plane normal to x/y/z                  */
+  int32_t src_pos;       /* `source_position` (x/y planes: even)                 */
+  int32_t src_quad;      /* z plane: 1 for (2,xx,yy,1), 2 for (2,2,xx,yy,1)      */
+  int32_t tt;            /* rows of source_waveform (tt,2)                       */
+  int32_t out_start, out_stop, out_step; /* `output_steps`                       */
+  float dt;              /* `dt`                                                 */
+  int32_t flags;         /* bit 0: use_reduced_precision (fp32 storage kept)     */
+  int32_t kernel;        /* PJZ_FDTD_KERNEL_*                                    */
+  int32_t tile_x, tile_y;/* launch overrides for the stream kernel, 0 = auto     */
+  int32_t reserved[4];
+} pjz_fdtd_problem;
+
+/* ABI version of the loaded library (PJZ_FDTD_ABI_VERSION). */
+PJZ_FDTD_API int pjz_fdtd_abi_version(void);
+
+/* Last error message of the calling thread ("" if none). */
+PJZ_FDTD_API const char* pjz_fdtd_last_error(void);
+
+/* Number of snapshots, len(range(*output_steps)). */
+PJZ_FDTD_API int pjz_fdtd_num_outputs(const pjz_fdtd_problem* p, int32_t* nn);
+
+/* Checks the problem and reports the workspace size for pjz_fdtd_forward. */
+PJZ_FDTD_API int pjz_fdtd_workspace_bytes(const pjz_fdtd_problem* p, size_t* bytes);
+
+/*
+ * Forward simulation: replaces fdtdz_jax.fdtdz(...) (src/pjz/_field.py:254-269).
+ *   epsilon          (3, ex, ey, ez)
+ *   absorption_mask  (3, xx, yy)
+ *   pml_kappa/sigma/alpha (zz, 2)
+ *   source_field     x: (2,1,yy,zz)  y: (2,xx,1,zz)  z: (src_quad,2,xx,yy,1)
+ *   source_waveform  (tt, 2)
+ *   out              (nn, 3, ex, ey, ez): E after step s, s in range(*output_steps)
+ */
+PJZ_FDTD_API int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
+                     const float* absorption_mask, const float* pml_kappa,
+                     const float* pml_sigma, const float* pml_alpha,
+                     const float* source_field, const float* source_waveform,
+                     float* out, void* workspace, size_t workspace_bytes,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PJZ_FDTD_H_ */

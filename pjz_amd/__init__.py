@@ -1,0 +1,13 @@
+"""MI355X-native fdtdz() Yee FDTD kernel behind pjz's call surface.
+
+``fdtdz`` is the drop-in for the kernel pjz calls at
+``src/pjz/_field.py:254-269``; ``boundaries`` holds the helpers that build its
+inputs and project its snapshots (``src/pjz/_field.py:56-150``).
+"""
+
+from .boundaries import (absorption_mask, frequency_components, output_phases,
+                         pml_sigma, ramped_sin, sampling_interval, source_amplitude)
+from .fdtdz_torch import fdtdz
+
+__all__ = ["fdtdz", "absorption_mask", "pml_sigma", "ramped_sin", "sampling_interval",
+           "output_phases", "frequency_components", "source_amplitude"]

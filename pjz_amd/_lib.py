@@ -1,0 +1,100 @@
+"""ctypes binding of libpjzfdtd.so (C ABI declared in include/pjz_fdtd.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+the library is missing, or no GPU is visible, the calls raise.
+"""
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpjzfdtd.so")
+ABI_VERSION = 1
+
+OK, EINVAL, EHIP, ESPACE = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_STREAM = 0, 1, 2
+
+# Every entry point include/pjz_fdtd.h declares.
+EXPORTS = (
+    "pjz_fdtd_abi_version",
+    "pjz_fdtd_last_error",
+    "pjz_fdtd_num_outputs",
+    "pjz_fdtd_workspace_bytes",
+    "pjz_fdtd_forward",
+)
+
+
+class Problem(ctypes.Structure):
+  """Mirror of ``pjz_fdtd_problem``."""
+  _fields_ = [
+      ("xx", ctypes.c_int32), ("yy", ctypes.c_int32), ("zz", ctypes.c_int32),
+      ("ex", ctypes.c_int32), ("ey", ctypes.c_int32), ("ez", ctypes.c_int32),
+      ("x0", ctypes.c_int32), ("y0", ctypes.c_int32), ("z0", ctypes.c_int32),
+      ("pml_lo", ctypes.c_int32), ("pml_hi", ctypes.c_int32),
+      ("src_axis", ctypes.c_int32), ("src_pos", ctypes.c_int32),
+      ("src_quad", ctypes.c_int32), ("tt", ctypes.c_int32),
+      ("out_start", ctypes.c_int32), ("out_stop", ctypes.c_int32),
+      ("out_step", ctypes.c_int32),
+      ("dt", ctypes.c_float),
+      ("flags", ctypes.c_int32), ("kernel", ctypes.c_int32),
+      ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32),
+      ("reserved", ctypes.c_int32 * 4),
+  ]
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load():
+  """Loads (once) and returns the library handle with typed entry points."""
+  global _lib
+  with _lock:
+    if _lib is not None:
+      return _lib
+    if not os.path.exists(LIB_PATH):
+      raise RuntimeError(
+          f"{LIB_PATH} is missing: build it with `python -m pjz_amd.build` "
+          "(the FDTD kernels have no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER(Problem)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    lib.pjz_fdtd_abi_version.restype = ctypes.c_int
+    lib.pjz_fdtd_abi_version.argtypes = []
+    lib.pjz_fdtd_last_error.restype = ctypes.c_char_p
+    lib.pjz_fdtd_last_error.argtypes = []
+    lib.pjz_fdtd_num_outputs.restype = ctypes.c_int
+    lib.pjz_fdtd_num_outputs.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
+    lib.pjz_fdtd_workspace_bytes.restype = ctypes.c_int
+    lib.pjz_fdtd_workspace_bytes.argtypes = [P, ctypes.POINTER(sz)]
+    lib.pjz_fdtd_forward.restype = ctypes.c_int
+    lib.pjz_fdtd_forward.argtypes = [P] + [vp] * 8 + [vp, sz, vp]
+    if lib.pjz_fdtd_abi_version() != ABI_VERSION:
+      raise RuntimeError("libpjzfdtd.so ABI version mismatch; rebuild it")
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+  """Raises ValueError / RuntimeError for a non-zero return code."""
+  if rc == OK:
+    return
+  msg = load().pjz_fdtd_last_error().decode(errors="replace")
+  if rc == EINVAL:
+    raise ValueError(f"{what}: {msg}")
+  raise RuntimeError(f"{what} failed (code {rc}): {msg}")
+
+
+def workspace_bytes(prob):
+  n = ctypes.c_size_t(0)
+  check(load().pjz_fdtd_workspace_bytes(ctypes.byref(prob), ctypes.byref(n)),
+        "pjz_fdtd_workspace_bytes")
+  return n.value
+
+
+def num_outputs(prob):
+  n = ctypes.c_int32(0)
+  check(load().pjz_fdtd_num_outputs(ctypes.byref(prob), ctypes.byref(n)),
+        "pjz_fdtd_num_outputs")
+  return n.value

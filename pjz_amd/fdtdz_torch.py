@@ -1,0 +1,170 @@
+"""``fdtdz()``: the drop-in for the kernel call pjz makes.
+
+pjz calls ``fdtdz_jax.fdtdz(**kwargs)`` at ``src/pjz/_field.py:254-269`` with
+exactly the keywords of :func:`fdtdz` below.  This module keeps that keyword
+set, the array shapes pjz passes and the ``(nn, 3, xx, yy, zz)`` return shape
+(``_field.py:189-191,276-279``), and runs the simulation through
+``libpjzfdtd.so`` on the current HIP stream.  Arrays may be torch tensors or
+numpy arrays; the result is a float32 torch tensor on the GPU.
+"""
+
+import ctypes
+from collections.abc import Mapping
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_KERNELS = {"auto": _lib.KERNEL_AUTO, "reference": _lib.KERNEL_REFERENCE,
+            "stream": _lib.KERNEL_STREAM}
+
+
+def _launch_options(launch_params):
+  """Parses ``launch_params``: None, a mapping, or a tuple of (key, value).
+
+  Keys: ``kernel`` ("auto" | "reference" | "stream"), ``tile_x``, ``tile_y``.
+  """
+  if launch_params is None:
+    return {}
+  if isinstance(launch_params, Mapping):
+    opts = dict(launch_params)
+  else:
+    try:
+      opts = dict(launch_params)
+    except (TypeError, ValueError):
+      raise ValueError(f"launch_params must be None, a mapping or (key, value) pairs, "
+                       f"got {launch_params!r}") from None
+  unknown = set(opts) - {"kernel", "tile_x", "tile_y"}
+  if unknown:
+    raise ValueError(f"unknown launch_params keys {sorted(unknown)}")
+  if "kernel" in opts and opts["kernel"] not in _KERNELS:
+    raise ValueError(f"launch_params kernel must be one of {sorted(_KERNELS)}")
+  return opts
+
+
+def source_geometry(source_shape, domain):
+  """(axis, quadratures) of a source plane from its shape; ValueError if bad.
+
+  x plane (2, 1, YY, ZZ), y plane (2, XX, 1, ZZ), z plane (2, 2, XX, YY, 1)
+  with two quadratures or (2, XX, YY, 1) with one (``_field.py:206-236``).
+  """
+  XX, YY, ZZ = domain
+  shp = tuple(int(s) for s in source_shape)
+  if len(shp) == 5:
+    if shp[0] not in (1, 2) or shp[1:] != (2, XX, YY, 1):
+      raise ValueError(f"z-plane source_field must be (2, 2, {XX}, {YY}, 1), got {shp}")
+    return 2, shp[0]
+  if len(shp) != 4 or shp[0] != 2:
+    raise ValueError(f"source_field must have shape (2, ...) with 4 dims or "
+                     f"(2, 2, ...) with 5 dims, got {shp}")
+  expect = {0: (2, 1, YY, ZZ), 1: (2, XX, 1, ZZ), 2: (2, XX, YY, 1)}
+  matches = [a for a in (0, 1, 2) if shp == expect[a]]
+  if not matches:
+    raise ValueError(f"source_field shape {shp} is none of {list(expect.values())}")
+  if len(matches) > 1:
+    raise ValueError(f"source_field shape {shp} is ambiguous for domain {domain}")
+  return matches[0], 1
+
+
+def make_problem(epsilon_shape, dt, source_shape, source_position, mask_shape,
+                 pml_shape, pml_widths, output_steps, tt, use_reduced_precision,
+                 launch_params, offset):
+  """Builds the C ``pjz_fdtd_problem`` from static arguments and shapes."""
+  eps_shape = tuple(int(s) for s in epsilon_shape)
+  if len(eps_shape) != 4 or eps_shape[0] != 3:
+    raise ValueError(f"epsilon must be (3, xx, yy, zz), got {eps_shape}")
+  mask_shape = tuple(int(s) for s in mask_shape)
+  if len(mask_shape) != 3 or mask_shape[0] != 3:
+    raise ValueError(f"absorption_mask must be (3, XX, YY), got {mask_shape}")
+  pml_shape = tuple(int(s) for s in pml_shape)
+  if len(pml_shape) != 2 or pml_shape[1] != 2:
+    raise ValueError(f"pml arrays must be (ZZ, 2), got {pml_shape}")
+  domain = (mask_shape[1], mask_shape[2], pml_shape[0])
+  axis, quad = source_geometry(source_shape, domain)
+  if len(tuple(output_steps)) != 3:
+    raise ValueError("output_steps must be (start, stop, step)")
+  if len(tuple(pml_widths)) != 2 or len(tuple(offset)) != 3:
+    raise ValueError("pml_widths must be (lo, hi) and offset (x0, y0, z0)")
+  opts = _launch_options(launch_params)
+  p = _lib.Problem()
+  p.xx, p.yy, p.zz = domain
+  p.ex, p.ey, p.ez = eps_shape[1:]
+  p.x0, p.y0, p.z0 = (int(v) for v in offset)
+  p.pml_lo, p.pml_hi = (int(v) for v in pml_widths)
+  p.src_axis, p.src_pos, p.src_quad = axis, int(source_position), quad
+  p.tt = int(tt)
+  p.out_start, p.out_stop, p.out_step = (int(v) for v in output_steps)
+  p.dt = float(dt)
+  p.flags = 1 if use_reduced_precision else 0
+  p.kernel = _KERNELS[opts.get("kernel", "auto")]
+  p.tile_x = int(opts.get("tile_x", 0))
+  p.tile_y = int(opts.get("tile_y", 0))
+  return p
+
+
+def _device_of(arrays):
+  for a in arrays:
+    if isinstance(a, torch.Tensor) and a.is_cuda:
+      return a.device
+  if not torch.cuda.is_available():
+    raise RuntimeError("fdtdz needs a HIP GPU: torch.cuda.is_available() is False")
+  return torch.device("cuda", torch.cuda.current_device())
+
+
+def _as_dev(a, device):
+  if isinstance(a, torch.Tensor):
+    t = a.detach()
+  else:
+    t = torch.as_tensor(np.asarray(a))
+  return t.to(device=device, dtype=torch.float32).contiguous()
+
+
+def _ptr(t):
+  return ctypes.c_void_p(t.data_ptr())
+
+
+def fdtdz(epsilon, dt, source_field, source_waveform, source_position,
+          absorption_mask, pml_kappa, pml_sigma, pml_alpha, pml_widths,
+          output_steps, use_reduced_precision, launch_params, offset=(0, 0, 0)):
+  """Runs the FDTD simulation; returns E snapshots (nn, 3, xx, yy, zz) float32.
+
+  Same keywords, shapes and meaning as the call at ``src/pjz/_field.py:254-269``.
+  Snapshot i is E after time step ``output_steps[0] + i * output_steps[2]``
+  over the epsilon window at ``offset``.  Raises ValueError for inconsistent
+  shapes or arguments, RuntimeError for GPU failures.
+  """
+  arrays = (epsilon, source_field, source_waveform, absorption_mask,
+            pml_kappa, pml_sigma, pml_alpha)
+  shapes = [tuple(np.shape(a)) if not isinstance(a, torch.Tensor) else tuple(a.shape)
+            for a in arrays]
+  for name, shp in zip(("pml_sigma", "pml_alpha"), shapes[5:]):
+    if shp != shapes[4]:
+      raise ValueError(f"{name} shape {shp} differs from pml_kappa {shapes[4]}")
+  wshape = shapes[2]
+  if len(wshape) != 2 or wshape[1] != 2:
+    raise ValueError(f"source_waveform must be (tt, 2), got {wshape}")
+  prob = make_problem(shapes[0], dt, shapes[1], source_position, shapes[3], shapes[4],
+                      pml_widths, output_steps, wshape[0], use_reduced_precision,
+                      launch_params, offset)
+  nn = _lib.num_outputs(prob)
+  device = _device_of(arrays)
+  lib = _lib.load()
+  out = torch.empty((nn, 3, prob.ex, prob.ey, prob.ez), dtype=torch.float32, device=device)
+  if nn == 0:
+    return out
+  dev = [_as_dev(a, device) for a in arrays]
+  nbytes = _lib.workspace_bytes(prob)
+  ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=device)
+  base = ws.data_ptr()
+  aligned = (base + 255) & ~255
+  with torch.cuda.device(device):
+    stream = torch.cuda.current_stream(device).cuda_stream
+    rc = lib.pjz_fdtd_forward(
+        ctypes.byref(prob), _ptr(dev[0]), _ptr(dev[3]), _ptr(dev[4]), _ptr(dev[5]),
+        _ptr(dev[6]), _ptr(dev[1]), _ptr(dev[2]), _ptr(out),
+        ctypes.c_void_p(aligned), ctypes.c_size_t(nbytes), ctypes.c_void_p(stream))
+  _lib.check(rc, "pjz_fdtd_forward")
+  # dev and ws were allocated on this stream, so the caching allocator only
+  # reuses their memory for work ordered after the kernels just enqueued.
+  return out

@@ -1,0 +1,91 @@
+"""Synthetic fdtdz() inputs for the benchmark and parity workloads.
+
+Each builder returns the keyword arguments of :func:`pjz_amd.fdtdz.fdtdz`
+(float32 numpy arrays plus static arguments), generated from fixed seeds.
+
+  layered_64     64x64x64, 2000 steps: SiO2 (eps 2.085) below z=24, a Si
+                 (12.25) strip 24<=z<32, 26<=y<38, air above, +-0.01 seeded
+                 noise; Ey Gaussian-pulse current plane at x=10 (centre
+                 frequency 2pi/20, width 60 steps, centred at step 240);
+                 z-CPML 10 cells/side (cubic sigma to 0.5, kappa 1, alpha
+                 0.05); absorber 10 cells; snapshots at steps 1990..1999.
+  native_512     512x512x128, 10000 steps: the same stack scaled (SiO2 below
+                 z=48, core slab 48<=z<64 with a 48-cell strip and a 200x200
+                 seeded Bernoulli(0.5) {2.085, 12.25} design region at
+                 x,y in [156,356)); seeded Gaussian yz mode-profile source at
+                 x=40 driven by pjz's ramped sine at 2pi/20; z-CPML 16 cells
+                 (pjz default sigma), absorber 32 cells; 8 snapshots.
+"""
+
+import math
+
+import numpy as np
+
+from . import boundaries as B
+
+
+def _pulse(tt, omega, t0, width):
+  t = np.arange(tt, dtype=np.float64)
+  return np.sin(omega * 0.5 * (t - t0)) * np.exp(-((t - t0) ** 2) / (2.0 * width ** 2))
+
+
+def layered_64(steps=2000, seed=0):
+  X = Y = Z = 64
+  rng = np.random.default_rng(seed)
+  eps = np.ones((3, X, Y, Z))
+  eps[:, :, :, :24] = 2.085
+  eps[:, :, 26:38, 24:32] = 12.25
+  eps += rng.uniform(-0.01, 0.01, eps.shape)
+  sf = np.zeros((2, 1, Y, Z))
+  sf[0] = 1.0  # Ey current over the whole x = 10 plane
+  wave = np.zeros((steps, 2))
+  wave[:, 0] = _pulse(steps, 2 * math.pi / 20, 240.0, 60.0)
+  sig = B.pml_sigma((10, 10), Z, 0.125, 3.0)
+  return dict(
+      epsilon=eps.astype(np.float32), dt=0.5, source_field=sf.astype(np.float32),
+      source_waveform=wave.astype(np.float32), source_position=10,
+      absorption_mask=B.absorption_mask(X, Y, 10, 2.5e-3),
+      pml_kappa=np.ones((Z, 2), np.float32), pml_sigma=sig,
+      pml_alpha=np.full((Z, 2), 0.05, np.float32), pml_widths=(10, 10),
+      output_steps=(steps - 10, steps, 3), use_reduced_precision=False,
+      launch_params=None, offset=(0, 0, 0))
+
+
+def native_512(steps=10000, seed=0, xx=512, yy=512, zz=128):
+  X, Y, Z = xx, yy, zz
+  rng = np.random.default_rng(seed)
+  zs = Z / 128.0
+  zc0, zc1 = int(48 * zs), int(64 * zs)
+  eps = np.ones((3, X, Y, Z), np.float32)
+  eps[:, :, :, :zc0] = 2.085
+  yc = Y // 2
+  eps[:, :, yc - 24:yc + 24, zc0:zc1] = 12.25
+  d0x, d0y = X // 2 - 100, Y // 2 - 100
+  design = np.where(rng.random((200, 200)) < 0.5, 2.085, 12.25).astype(np.float32)
+  eps[:, d0x:d0x + 200, d0y:d0y + 200, zc0:zc1] = design[None, :, :, None]
+  yy_, zz_ = np.meshgrid(np.arange(Y) - yc, np.arange(Z) - (zc0 + zc1) / 2, indexing="ij")
+  prof = np.exp(-(yy_ ** 2) / (2 * 16.0 ** 2) - (zz_ ** 2) / (2 * 6.0 ** 2))
+  prof *= 1 + 0.05 * rng.standard_normal(prof.shape)
+  sf = np.zeros((2, 1, Y, Z), np.float32)
+  sf[0, 0] = prof
+  rs = B.ramped_sin(np.array([2 * math.pi / 20]), 4.0, 4.0, 0.5, steps)
+  wave = np.zeros((steps, 2), np.float32)
+  wave[:, 0] = np.imag(rs)
+  return dict(
+      epsilon=eps, dt=0.5, source_field=sf, source_waveform=wave, source_position=40,
+      absorption_mask=B.absorption_mask(X, Y, 32, 2.5e-4),
+      pml_kappa=np.ones((Z, 2), np.float32), pml_sigma=B.pml_sigma((16, 16), Z, 0.5, 1.3),
+      pml_alpha=np.zeros((Z, 2), np.float32), pml_widths=(16, 16),
+      output_steps=(steps - 24, steps, 3), use_reduced_precision=True,
+      launch_params=None, offset=(0, 0, 0))
+
+
+def cells(kw):
+  m = np.shape(kw["absorption_mask"])
+  return m[1] * m[2] * np.shape(kw["pml_sigma"])[0]
+
+
+def time_steps(kw):
+  """Number of FDTD steps fdtdz() runs: the last snapshot step + 1."""
+  s = range(*kw["output_steps"])
+  return s[-1] + 1 if len(s) else 0

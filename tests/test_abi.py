@@ -1,0 +1,87 @@
+"""CPU checks of the C ABI: the library loads, exports what the header
+declares, and rejects bad problems with ValueError -- no GPU calls."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from pjz_amd import _lib
+from pjz_amd import fdtdz_torch as F
+from pjz_amd import workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+  names = []
+  for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+    if fn.endswith(".h"):
+      text = open(os.path.join(ROOT, "include", fn)).read()
+      names += re.findall(r"PJZ_FDTD_API\s+[\w\s\*]*?\b(pjz_fdtd_\w+)\s*\(", text)
+  return names
+
+
+def test_header_declares_entry_points():
+  names = declared_symbols()
+  assert set(names) == set(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+  lib = ctypes.CDLL(_lib.LIB_PATH)
+  for name in declared_symbols():
+    assert hasattr(lib, name), name
+  assert _lib.load().pjz_fdtd_abi_version() == _lib.ABI_VERSION
+
+
+def _problem(**over):
+  kw = workloads.layered_64(steps=40)
+  kw.update(over)
+  shapes = {k: np.shape(v) for k, v in kw.items() if isinstance(v, np.ndarray)}
+  return F.make_problem(shapes["epsilon"], kw["dt"], shapes["source_field"],
+                        kw["source_position"], shapes["absorption_mask"],
+                        shapes["pml_kappa"], kw["pml_widths"], kw["output_steps"],
+                        shapes["source_waveform"][0], kw["use_reduced_precision"],
+                        kw["launch_params"], kw["offset"])
+
+
+def test_workspace_and_outputs():
+  p = _problem()
+  assert _lib.num_outputs(p) == 4
+  n = _lib.workspace_bytes(p)
+  cells = 64 ** 3
+  assert n >= 4 * 5 * 3 * cells  # E, H ping-pong + cb
+
+
+@pytest.mark.parametrize("over, msg", [
+    (dict(source_position=11), "even"),
+    (dict(output_steps=(0, 100, 1)), "waveform rows"),
+    (dict(output_steps=(0, 10, 0)), "step"),
+    (dict(pml_widths=(40, 40)), "pml_widths"),
+    (dict(offset=(1, 0, 0)), "does not fit"),
+    (dict(dt=0.0), "dt"),
+])
+def test_bad_problem_raises_value_error(over, msg):
+  with pytest.raises(ValueError, match=msg):
+    _lib.workspace_bytes(_problem(**over))
+
+
+def test_bad_shapes_raise_value_error():
+  with pytest.raises(ValueError, match="source_field"):
+    F.source_geometry((2, 3, 64, 64), (64, 64, 64))
+  with pytest.raises(ValueError, match="ambiguous"):
+    F.source_geometry((2, 1, 1, 8), (1, 1, 8))
+  assert F.source_geometry((2, 2, 8, 9, 1), (8, 9, 10)) == (2, 2)
+  assert F.source_geometry((2, 8, 9, 1), (8, 9, 10)) == (2, 1)
+  assert F.source_geometry((2, 8, 1, 10), (8, 9, 10)) == (1, 1)
+  with pytest.raises(ValueError, match="launch_params"):
+    F._launch_options(dict(kernel="nope"))
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU failure")
+def test_fdtdz_fails_loudly_without_gpu():
+  with pytest.raises(RuntimeError, match="GPU"):
+    F.fdtdz(**workloads.layered_64(steps=20))

@@ -1,0 +1,144 @@
+"""GPU parity of fdtdz() (through the C ABI) against the FP64 oracle.
+
+Tolerance: relative L2 error <= 1e-4 over all snapshots, FP32 GPU fields vs
+the FP64 oracle, for every kernel.  Full-size (512x512x128) runs are checked
+through properties the oracle cannot afford: kernel-vs-kernel agreement,
+linearity in the waveform and exact time-shift invariance.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle as C
+from oracle import helpers as H
+from pjz_amd import fdtdz_torch as F
+from pjz_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = ["reference"]
+TOL = 1e-4
+
+
+def rel_l2(a, b):
+  a = np.asarray(a, np.float64)
+  b = np.asarray(b, np.float64)
+  return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_gpu(kw, kernel, **over):
+  kw = dict(kw, **over)
+  kw["launch_params"] = dict(kernel=kernel)
+  out = F.fdtdz(**kw)
+  torch.cuda.synchronize()
+  return out.cpu().numpy()
+
+
+def random_problem(axis, X=24, Y=20, Z=32, tt=120, seed=0, widths=(6, 5)):
+  rng = np.random.default_rng(seed)
+  ex, ey, ez = X - 6, Y - 4, Z - 10
+  eps = (1 + 4 * rng.random((3, ex, ey, ez))).astype(np.float32)
+  sf = {0: rng.standard_normal((2, 1, Y, Z)), 1: rng.standard_normal((2, X, 1, Z)),
+        2: rng.standard_normal((2, 2, X, Y, 1))}[axis].astype(np.float32)
+  pos = {0: 8, 1: 6, 2: 13}[axis]
+  wave = np.zeros((tt, 2), np.float32)
+  t = np.arange(tt)
+  wave[:, 0] = np.sin(0.3 * t) * np.exp(-((t - 30) / 12.0) ** 2)
+  wave[:, 1] = np.cos(0.3 * t) * np.exp(-((t - 30) / 12.0) ** 2)
+  return dict(epsilon=eps, dt=0.5, source_field=sf, source_waveform=wave,
+              source_position=pos, absorption_mask=H.absorption_mask(X, Y, 5, 4e-3).astype(
+                  np.float32),
+              pml_kappa=(1 + rng.random((Z, 2))).astype(np.float32),
+              pml_sigma=H.pml_sigma(widths, Z, 0.5, 3.0).astype(np.float32),
+              pml_alpha=np.full((Z, 2), 0.05, np.float32), pml_widths=widths,
+              output_steps=(tt - 40, tt, 7), use_reduced_precision=False, launch_params=None,
+              offset=(3, 2, 5))
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_random_problem_matches_oracle(kernel, axis):
+  print(f"case kernel={kernel} axis={axis}", flush=True)
+  kw = random_problem(axis, seed=axis)
+  ref = C.fdtdz(**kw)
+  got = run_gpu(kw, kernel)
+  assert got.shape == ref.shape
+  assert rel_l2(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("shape", [(17, 13, 9), (10, 9, 64), (9, 11, 96), (8, 7, 128),
+                                   (6, 5, 200)])
+def test_odd_shapes_match_oracle(kernel, shape):
+  X, Y, Z = shape
+  print(f"case kernel={kernel} shape={shape}", flush=True)
+  rng = np.random.default_rng(X * Y)
+  kw = random_problem(2, X=20, Y=20, Z=20, tt=50, seed=X * Y, widths=(2, 3))
+  kw.update(epsilon=(1 + 4 * rng.random((3, X, Y, Z))).astype(np.float32),
+            source_field=rng.standard_normal((2, 2, X, Y, 1)).astype(np.float32),
+            source_position=Z // 2, offset=(0, 0, 0),
+            absorption_mask=(0.05 * rng.random((3, X, Y))).astype(np.float32),
+            pml_kappa=np.ones((Z, 2), np.float32),
+            pml_sigma=H.pml_sigma((2, 3), Z, 0.5, 3.0).astype(np.float32),
+            pml_alpha=np.full((Z, 2), 0.05, np.float32))
+  ref = C.fdtdz(**kw)
+  assert rel_l2(run_gpu(kw, kernel), ref) <= TOL
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_z_as_batch_matches_oracle(kernel):
+  kw = random_problem(1, seed=7)
+  Z = kw["pml_sigma"].shape[0]
+  kw["pml_kappa"] = np.full((Z, 2), np.inf, np.float32)
+  kw["pml_sigma"] = np.zeros((Z, 2), np.float32)
+  kw["pml_alpha"] = np.zeros((Z, 2), np.float32)
+  assert rel_l2(run_gpu(kw, kernel), C.fdtdz(**kw)) <= TOL
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_layered_64_full_run_matches_oracle(kernel):
+  """64x64x64, 2000 steps, snapshots 1990..1999 (the CPU-runnable workload)."""
+  kw = workloads.layered_64()
+  ref = C.fdtdz(**kw)
+  got = run_gpu(kw, kernel)
+  err = rel_l2(got, ref)
+  print(f"layered_64 {kernel}: rel L2 {err:.3e}, |ref| {np.abs(ref).max():.3e}")
+  assert err <= TOL
+
+
+def test_native_512_short_run_matches_oracle():
+  """512x512x128 at its real shape for 60 steps against the oracle."""
+  kw = workloads.native_512(steps=60)
+  ref = C.fdtdz(**kw)
+  for kernel in KERNELS:
+    err = rel_l2(run_gpu(kw, kernel), ref)
+    print(f"native_512 60 steps {kernel}: rel L2 {err:.3e}")
+    assert err <= TOL
+
+
+def test_time_shift_invariance_is_exact():
+  kw = random_problem(0, tt=100, seed=3)
+  shift = 17
+  w = kw["source_waveform"]
+  kw2 = dict(kw, source_waveform=np.concatenate([np.zeros((shift, 2), np.float32), w]),
+             output_steps=tuple(s + shift if i < 2 else s
+                                for i, s in enumerate(kw["output_steps"])))
+  for kernel in KERNELS:
+    a = run_gpu(kw, kernel)
+    b = run_gpu(kw2, kernel)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_zero_source_gives_zero_fields():
+  kw = random_problem(2, seed=4)
+  kw["source_waveform"] = np.zeros_like(kw["source_waveform"])
+  for kernel in KERNELS:
+    assert np.abs(run_gpu(kw, kernel)).max() == 0.0
+
+
+def test_empty_output_range():
+  kw = random_problem(0, seed=5)
+  kw["output_steps"] = (50, 50, 1)
+  out = F.fdtdz(**kw)
+  assert tuple(out.shape) == (0, 3) + kw["epsilon"].shape[1:]

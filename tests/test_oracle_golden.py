@@ -1,0 +1,108 @@
+"""Pins the oracle's helper restatements with the reference's own vectors.
+
+Expected values are the literal vectors and properties of the reference tests
+tests/test_boundaries.py:8-41, tests/test_waveforms.py:8-14 and
+tests/test_frequencies.py:8-38, re-targeted at the oracle's names.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import helpers as H
+
+
+def test_absorption_mask_rows():  # tests/test_boundaries.py:8-24
+  m = H.absorption_mask(xx=10, yy=10, width=3, smoothness=4)
+  np.testing.assert_array_equal(m[0, :, 5], [36, 16, 4, 0, 0, 0, 0, 4, 16, 36])
+  np.testing.assert_array_equal(m[0, 5, :], [25, 9, 1, 0, 0, 0, 1, 9, 25, 49])
+  np.testing.assert_array_equal(m[1, :, 5], [25, 9, 1, 0, 0, 0, 1, 9, 25, 49])
+  np.testing.assert_array_equal(m[1, 5, :], [36, 16, 4, 0, 0, 0, 0, 4, 16, 36])
+  np.testing.assert_array_equal(m[2, :, 5], [25, 9, 1, 0, 0, 0, 1, 9, 25, 49])
+  np.testing.assert_array_equal(m[2, 5, :], [25, 9, 1, 0, 0, 0, 1, 9, 25, 49])
+
+
+PML_TABLE = [[8.0000000e+01, 4.6894531e+01], [2.5312500e+01, 1.2207031e+01],
+             [5.0000000e+00, 1.5820312e+00], [3.1250000e-01, 1.9531250e-02],
+             [0.0, 0.0], [0.0, 0.0],
+             [1.9531250e-02, 3.1250000e-01], [1.5820312e+00, 5.0000000e+00],
+             [1.2207031e+01, 2.5312500e+01], [4.6894531e+01, 8.0000000e+01]]
+
+
+def test_pml_sigma_table():  # tests/test_boundaries.py:27-41
+  np.testing.assert_array_almost_equal(
+      H.pml_sigma(pml_widths=(4, 4), zz_=10, ln_R=16.0, m=4.0), PML_TABLE)
+
+
+def test_pml_sigma_zero_width():
+  s = H.pml_sigma((0, 3), 8, 0.5, 2.0)
+  assert np.all(s[:5] == 0) and np.all(s[5:] > 0)
+
+
+def test_ramped_sin_asymptote():  # tests/test_waveforms.py:8-14
+  omega, dt, tt = 2 * np.pi / 40, 0.5, 100000
+  w = H.ramped_sin(omega, width=3, delay=4, dt=dt, tt=tt)
+  np.testing.assert_array_almost_equal(
+      np.imag(w)[-10:], np.sin(omega * dt * np.arange(tt))[-10:])
+
+
+def test_sampling_interval_orthogonal():  # tests/test_frequencies.py:8-18
+  wmin, wmax, n, dt = 2 * np.pi / 40, 2 * np.pi / 36, 10, 0.5
+  interval = H.sampling_interval(wmin, wmax, n, dt)
+  ws = np.linspace(wmin, wmax, n)
+  theta = ws * dt * interval * np.arange(2 * n)[:, None]
+  A = np.hstack([np.sin(theta), np.cos(theta)])
+  A /= np.linalg.norm(A, ord=2, axis=0)
+  assert np.all(A.T @ A - np.eye(2 * n) < 1e-1)
+
+
+def test_sampling_interval_single():
+  # one frequency: a quarter period, _field.py:126-128
+  assert H.sampling_interval(2 * np.pi / 20, 2 * np.pi / 20, 1, 0.5) == 10
+
+
+def test_frequency_components_phase_convention():  # tests/test_frequencies.py:21-29
+  omegas = np.array([2 * np.pi / 40, 2 * np.pi / 39])
+  steps = np.array([10, 20, 30, 40])
+  dt = 0.5
+  signal = np.sin(omegas[0] * dt * steps) + np.cos(omegas[1] * dt * steps)
+  out = H.frequency_components(signal[:, None, None, None, None], steps, omegas, dt)
+  assert out[0] == pytest.approx(-1j, rel=1e-3)
+  assert out[1] == pytest.approx(1, rel=1e-3)
+
+
+def test_frequency_components_exact_multitone():
+  rng = np.random.default_rng(3)
+  omegas = np.array([2 * np.pi / 22, 2 * np.pi / 20, 2 * np.pi / 18])
+  dt, interval = 0.5, H.sampling_interval(omegas.min(), omegas.max(), 3, 0.5)
+  steps = np.arange(1000, 1000 + 7 * interval, interval)
+  amp = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+  t = omegas[:, None] * dt * steps[None, :]
+  sig = np.real(amp[:, None] * np.exp(1j * t)).sum(0)
+  np.testing.assert_allclose(H.frequency_components(sig, steps, omegas, dt), amp,
+                             rtol=1e-9, atol=1e-9)
+
+
+def test_source_amplitude_half_step():  # tests/test_frequencies.py:32-38
+  omega, dt, tt = 2 * np.pi / 40, 0.5, 10000
+  out = H.source_amplitude(np.sin(omega * dt * (np.arange(tt) - 0.5))[:, None], omega, dt)
+  assert out[0] == pytest.approx(-1j)
+
+
+def test_pad_zz_identities():
+  for ez in (1, 7, 24, 50, 96):
+    for pml in ((16, 16), (10, 6), (0, 0)):
+      for red in (True, False):
+        total = H.zz(pml, red)
+        if ez > total:
+          continue
+        bot, top = H.pad_zz(ez, pml, red)
+        assert bot + top + ez == total and 0 <= top - bot <= 1
+  assert H.zz((16, 16), True) == 96 and H.zz((16, 16), False) == 32
+
+
+def test_prop_axis():
+  assert H.prop_axis((3, 2, 1, 5, 6)) == "x"
+  assert H.prop_axis((2, 5, 1, 6)) == "y"
+  assert H.prop_axis((2, 5, 6, 1)) == "z"
+  with pytest.raises(ValueError):
+    H.prop_axis((2, 1, 1, 6))
