@@ -113,7 +113,7 @@ def main():
   ms_per_step = wall * 1e3 / args.steps
   value = cells * tsteps * args.steps / wall / 1e9
 
-  kernel_name = "reference" if args.kernel == "auto" else args.kernel
+  kernel_name = "stream" if args.kernel == "auto" else args.kernel
   launches_per_step = 1  # stepper launches per FDTD step (reference: H+E pair)
   bpc = algorithmic_bytes_per_cell(zz, lo, hi)
   bytes_per_launch = cells * bpc

@@ -70,7 +70,8 @@ typedef struct pjz_fdtd_problem {
   int32_t flags;         /* bit 0: use_reduced_precision (fp32 storage kept)     */
   int32_t kernel;        /* PJZ_FDTD_KERNEL_*                                    */
   int32_t tile_x, tile_y;/* launch overrides for the stream kernel, 0 = auto     */
-  int32_t reserved[4];
+  int32_t waves_per_simd;/* occupancy hint for the stream kernel, 0 = auto      */
+  int32_t reserved[3];
 } pjz_fdtd_problem;
 
 /* ABI version of the loaded library (PJZ_FDTD_ABI_VERSION). */

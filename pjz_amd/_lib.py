@@ -39,7 +39,7 @@ class Problem(ctypes.Structure):
       ("dt", ctypes.c_float),
       ("flags", ctypes.c_int32), ("kernel", ctypes.c_int32),
       ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32),
-      ("reserved", ctypes.c_int32 * 4),
+      ("waves_per_simd", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
   ]
 
 

@@ -72,6 +72,8 @@ int check(const pjz_fdtd_problem* p) {
   }
   if (p->kernel < PJZ_FDTD_KERNEL_AUTO || p->kernel > PJZ_FDTD_KERNEL_STREAM)
     return fail(PJZ_FDTD_EINVAL, "unknown kernel id %d", p->kernel);
+  if (p->kernel == PJZ_FDTD_KERNEL_STREAM && p->zz > 256)
+    return fail(PJZ_FDTD_EINVAL, "the stream kernel needs zz <= 256, got %d", p->zz);
   return PJZ_FDTD_OK;
 }
 
@@ -142,7 +144,9 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
   pjz::Src src{p->src_axis, p->src_pos, p->src_quad, source_field, source_waveform};
   pjz::OutWin win{p->x0, p->y0, p->z0, p->ex, p->ey, p->ez, out};
   int nsteps = p->out_start + (nn - 1) * p->out_step + 1;
-  int kernel = p->kernel == PJZ_FDTD_KERNEL_AUTO ? PJZ_FDTD_KERNEL_REFERENCE : p->kernel;
+  int kernel = p->kernel;
+  if (kernel == PJZ_FDTD_KERNEL_AUTO)
+    kernel = g.Z <= 256 ? PJZ_FDTD_KERNEL_STREAM : PJZ_FDTD_KERNEL_REFERENCE;
   if (kernel == PJZ_FDTD_KERNEL_STREAM)
     e = pjz::run_stream(g, *p, absorption_mask, src, win, ws, L, nsteps, s);
   else

@@ -18,7 +18,11 @@ enum ZRow { IKE = 0, BE = 1, CE = 2, IKH = 3, BH = 4, CH = 5, ZROWS = 8 };
 // psi components, each (xx, yy, np) with np = pml_lo + pml_hi:
 //   PSI_HX: d/dz Ey in the Hx update   PSI_HY: d/dz Ex in the Hy update
 //   PSI_EX: d/dz Hy in the Ex update   PSI_EY: d/dz Hx in the Ey update
-enum PsiComp { PSI_HX = 0, PSI_HY = 1, PSI_EX = 2, PSI_EY = 3, PSI_N = 4 };
+//   PSI_HX2, PSI_HY2: second buffer of the H pair.  The streaming kernel
+//   recomputes H on ghost rows/planes owned by other workgroups, so it reads
+//   psi_H from one buffer and writes the other (like E and H themselves).
+enum PsiComp { PSI_HX = 0, PSI_HY = 1, PSI_EX = 2, PSI_EY = 3, PSI_HX2 = 4, PSI_HY2 = 5,
+               PSI_N = 6 };
 
 struct Geom {
   int X, Y, Z;      // domain

@@ -23,7 +23,8 @@ _KERNELS = {"auto": _lib.KERNEL_AUTO, "reference": _lib.KERNEL_REFERENCE,
 def _launch_options(launch_params):
   """Parses ``launch_params``: None, a mapping, or a tuple of (key, value).
 
-  Keys: ``kernel`` ("auto" | "reference" | "stream"), ``tile_x``, ``tile_y``.
+  Keys: ``kernel`` ("auto" | "reference" | "stream"), ``tile_x``, ``tile_y``,
+  ``waves_per_simd``.
   """
   if launch_params is None:
     return {}
@@ -35,7 +36,7 @@ def _launch_options(launch_params):
     except (TypeError, ValueError):
       raise ValueError(f"launch_params must be None, a mapping or (key, value) pairs, "
                        f"got {launch_params!r}") from None
-  unknown = set(opts) - {"kernel", "tile_x", "tile_y"}
+  unknown = set(opts) - {"kernel", "tile_x", "tile_y", "waves_per_simd"}
   if unknown:
     raise ValueError(f"unknown launch_params keys {sorted(unknown)}")
   if "kernel" in opts and opts["kernel"] not in _KERNELS:
@@ -100,6 +101,7 @@ def make_problem(epsilon_shape, dt, source_shape, source_position, mask_shape,
   p.kernel = _KERNELS[opts.get("kernel", "auto")]
   p.tile_x = int(opts.get("tile_x", 0))
   p.tile_y = int(opts.get("tile_y", 0))
+  p.waves_per_simd = int(opts.get("waves_per_simd", 0))
   return p
 
 

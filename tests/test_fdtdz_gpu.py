@@ -17,7 +17,7 @@ from pjz_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = ["reference"]
+KERNELS = ["reference", "stream"]
 TOL = 1e-4
 
 
@@ -142,3 +142,14 @@ def test_empty_output_range():
   kw["output_steps"] = (50, 50, 1)
   out = F.fdtdz(**kw)
   assert tuple(out.shape) == (0, 3) + kw["epsilon"].shape[1:]
+
+
+@pytest.mark.parametrize("tile_x", [7, 32])
+@pytest.mark.parametrize("tile_y", [2, 4, 8, 12, 15])
+def test_stream_launch_variants_match_oracle(tile_x, tile_y):
+  kw = random_problem(2, X=40, Y=30, Z=64 + 32, tt=60, seed=11, widths=(8, 8))
+  kw["offset"] = (3, 2, 5)
+  ref = C.fdtdz(**kw)
+  kw["launch_params"] = dict(kernel="stream", tile_x=tile_x, tile_y=tile_y)
+  got = F.fdtdz(**kw).cpu().numpy()
+  assert rel_l2(got, ref) <= TOL
