@@ -48,20 +48,32 @@ def cpu_baseline(kw, target_s=10.0):
     threads = os.cpu_count() or 1
   threads = max(1, min(threads, 16))
   cells = workloads.cells(kw)
-  probe = 3
-  t0 = time.perf_counter()
-  c_oracle.fdtdz(**kw, threads=threads, nsteps=probe)
-  t_probe = time.perf_counter() - t0
-  n = int(max(probe, min(400, probe * target_s / max(t_probe, 1e-3))))
-  t0 = time.perf_counter()
-  c_oracle.fdtdz(**kw, threads=threads, nsteps=n)
-  wall = time.perf_counter() - t0
+
+  def run(n):
+    t0 = time.perf_counter()
+    c_oracle.fdtdz(**kw, threads=threads, nsteps=n)
+    return time.perf_counter() - t0
+
+  # Set-up (coefficients, allocation) is timed at 0 steps and subtracted.
+  t_setup = run(0)
+  t_probe = run(3) - t_setup
+  n = int(max(3, min(2000, 3 * target_s / max(t_probe, 1e-3))))
+  wall = run(n) - t_setup
   return {"value": round(cells * n / wall / 1e9, 5), "unit": "Gcell-updates/s",
           "cores": threads, "kind": "port",
           "sample": f"FP64 C oracle (oracle/fdtd_ref.c, OpenMP) on the full "
                     f"{kw['epsilon'].shape[1]}x{kw['epsilon'].shape[2]}x"
-                    f"{kw['epsilon'].shape[3]} grid for {n} steps incl. set-up, "
-                    f"{wall:.1f} s wall"}
+                    f"{kw['epsilon'].shape[3]} grid, {n} steps in {wall:.1f} s "
+                    f"(set-up of {t_setup:.1f} s excluded)"}
+
+
+def measured_traffic(kernel, workload):
+  """HBM bytes per launch from the committed rocprofv3 counter summary."""
+  path = os.path.join(ROOT, "profiles", f"k_{kernel}_{workload}.json")
+  if not os.path.exists(path):
+    return None
+  with open(path) as f:
+    return json.load(f)["traffic_bytes_per_launch"]
 
 
 def main():
@@ -133,7 +145,8 @@ def main():
                  "temporal_block": 1, "pml_widths": [int(lo), int(hi)]},
       "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                   "traffic": None,
+                   "traffic": measured_traffic(kernel_name, args.workload),
+                   "bytes_per_launch": round(bytes_per_launch),
                    "algorithmic_bytes_per_cell": round(bpc, 4),
                    "avg_launch_us": round(avg_launch_s * 1e6, 2)},
   }
