@@ -71,7 +71,9 @@ typedef struct pjz_fdtd_problem {
   int32_t kernel;        /* PJZ_FDTD_KERNEL_*                                    */
   int32_t tile_x, tile_y;/* launch overrides for the stream kernel, 0 = auto     */
   int32_t waves_per_simd;/* occupancy hint for the stream kernel, 0 = auto      */
-  int32_t reserved[3];
+  int32_t ckpt_every;    /* > 0: forward saves a checkpoint every ckpt_every     */
+                         /* steps so pjz_fdtd_backward can run on the workspace  */
+  int32_t reserved[2];
 } pjz_fdtd_problem;
 
 /* ABI version of the loaded library (PJZ_FDTD_ABI_VERSION). */
@@ -101,6 +103,24 @@ PJZ_FDTD_API int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilo
                      const float* source_field, const float* source_waveform,
                      float* out, void* workspace, size_t workspace_bytes,
                      void* stream);
+
+/*
+ * Gradient of sum_s <grad_out[s], out[s]> with respect to epsilon, by the
+ * discrete adjoint of the update (reverse time, transposed curls, CPML and
+ * absorber), segment by segment from the checkpoints the preceding
+ * pjz_fdtd_forward call saved in the same workspace (p->ckpt_every > 0, same
+ * problem and arrays).  No counterpart in pjz, which never differentiates
+ * through the kernel (scatter() supplies its own VJP, src/pjz/_field.py:393-398);
+ * it is the addition inverse design needs.
+ *   grad_out   (nn, 3, ex, ey, ez) cotangent of out
+ *   grad_eps   (3, ex, ey, ez), overwritten
+ */
+PJZ_FDTD_API int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
+                      const float* absorption_mask, const float* pml_kappa,
+                      const float* pml_sigma, const float* pml_alpha,
+                      const float* source_field, const float* source_waveform,
+                      const float* grad_out, float* grad_eps, void* workspace,
+                      size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@ EXPORTS = (
     "pjz_fdtd_num_outputs",
     "pjz_fdtd_workspace_bytes",
     "pjz_fdtd_forward",
+    "pjz_fdtd_backward",
 )
 
 
@@ -39,7 +40,8 @@ class Problem(ctypes.Structure):
       ("dt", ctypes.c_float),
       ("flags", ctypes.c_int32), ("kernel", ctypes.c_int32),
       ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32),
-      ("waves_per_simd", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+      ("waves_per_simd", ctypes.c_int32), ("ckpt_every", ctypes.c_int32),
+      ("reserved", ctypes.c_int32 * 2),
   ]
 
 
@@ -70,6 +72,8 @@ def load():
     lib.pjz_fdtd_workspace_bytes.argtypes = [P, ctypes.POINTER(sz)]
     lib.pjz_fdtd_forward.restype = ctypes.c_int
     lib.pjz_fdtd_forward.argtypes = [P] + [vp] * 8 + [vp, sz, vp]
+    lib.pjz_fdtd_backward.restype = ctypes.c_int
+    lib.pjz_fdtd_backward.argtypes = [P] + [vp] * 9 + [vp, sz, vp]
     if lib.pjz_fdtd_abi_version() != ABI_VERSION:
       raise RuntimeError("libpjzfdtd.so ABI version mismatch; rebuild it")
     _lib = lib

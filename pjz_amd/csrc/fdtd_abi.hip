@@ -89,6 +89,64 @@ pjz::Geom geom_of(const pjz_fdtd_problem& p) {
   return g;
 }
 
+// Steps [b, end) with the kernel the problem selects (AUTO: stream if zz <= 256).
+hipError_t run_steps(const pjz::Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                     const pjz::Src& src, const pjz::OutWin& win, char* ws,
+                     const pjz::Layout& L, int b, int end, int* cur, float* capture,
+                     hipStream_t s) {
+  int kernel = p.kernel;
+  if (kernel == PJZ_FDTD_KERNEL_AUTO)
+    kernel = g.Z <= 256 ? PJZ_FDTD_KERNEL_STREAM : PJZ_FDTD_KERNEL_REFERENCE;
+  pjz::StepRange r{b, end, cur, capture};
+  if (kernel == PJZ_FDTD_KERNEL_STREAM) return pjz::run_stream(g, p, mask, src, win, ws, L, r, s);
+  return pjz::run_reference(g, p, mask, src, win, ws, L, r, s);
+}
+
+// Checkpoint k holds E, H, the psi_H pair and the psi_E pair of the state that
+// lives in ping-pong set `cur`.
+struct StateParts {
+  float* E;
+  float* H;
+  float* psiH;
+  float* psiE;
+};
+
+StateParts state_parts(const pjz::Geom& g, const pjz::Layout& L, char* ws, int cur) {
+  float* psi = (float*)(ws + L.psi);
+  const long long P = (long long)g.X * g.Y * g.NP;
+  return {(float*)(ws + (cur ? L.E1 : L.E0)), (float*)(ws + (cur ? L.H1 : L.H0)),
+          psi + (cur ? pjz::PSI_HX2 : pjz::PSI_HX) * P, psi + pjz::PSI_EX * P};
+}
+
+hipError_t copy_state(const pjz::Geom& g, const pjz::Layout& L, char* ws, int cur, int k,
+                      bool save, hipStream_t s) {
+  StateParts st = state_parts(g, L, ws, cur);
+  float* c = (float*)(ws + L.ckpt + pjz::ckpt_stride(L) * (size_t)k);
+  const size_t n3 = 3 * (size_t)g.N, p2 = 2 * (size_t)g.X * g.Y * (size_t)g.NP;
+  float* parts[4] = {st.E, st.H, st.psiH, st.psiE};
+  const size_t sizes[4] = {n3, n3, p2, p2};
+  size_t off = 0;
+  for (int i = 0; i < 4; ++i) {
+    if (sizes[i] == 0) continue;
+    hipError_t e = save ? pjz::launch_copy(c + off, parts[i], sizes[i], s)
+                        : pjz::launch_copy(parts[i], c + off, sizes[i], s);
+    if (e != hipSuccess) return e;
+    off += sizes[i];
+  }
+  return hipSuccess;
+}
+
+hipError_t save_state(const pjz::Geom& g, const pjz::Layout& L, char* ws, int cur, int k,
+                      hipStream_t s) {
+  return copy_state(g, L, ws, cur, k, true, s);
+}
+
+// Restores checkpoint k into ping-pong set 0.
+hipError_t restore_state(const pjz::Geom& g, const pjz::Layout& L, char* ws, int k,
+                         hipStream_t s) {
+  return copy_state(g, L, ws, 0, k, false, s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -109,7 +167,7 @@ int pjz_fdtd_workspace_bytes(const pjz_fdtd_problem* p, size_t* bytes) {
   int rc = check(p);
   if (rc) return rc;
   if (!bytes) return fail(PJZ_FDTD_EINVAL, "bytes is NULL");
-  *bytes = pjz::make_layout(geom_of(*p)).total;
+  *bytes = pjz::make_layout(geom_of(*p), *p).total;
   return PJZ_FDTD_OK;
 }
 
@@ -126,7 +184,7 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
       !source_waveform || !out || !workspace)
     return fail(PJZ_FDTD_EINVAL, "a required pointer argument is NULL");
   pjz::Geom g = geom_of(*p);
-  pjz::Layout L = pjz::make_layout(g);
+  pjz::Layout L = pjz::make_layout(g, *p);
   if (workspace_bytes < L.total)
     return fail(PJZ_FDTD_ESPACE, "workspace of %zu bytes, %zu needed", workspace_bytes, L.total);
   if (((uintptr_t)workspace & 255) != 0)
@@ -143,15 +201,83 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
   if (e != hipSuccess) return hip_fail(e, "clear state");
   pjz::Src src{p->src_axis, p->src_pos, p->src_quad, source_field, source_waveform};
   pjz::OutWin win{p->x0, p->y0, p->z0, p->ex, p->ey, p->ez, out};
-  int nsteps = p->out_start + (nn - 1) * p->out_step + 1;
-  int kernel = p->kernel;
-  if (kernel == PJZ_FDTD_KERNEL_AUTO)
-    kernel = g.Z <= 256 ? PJZ_FDTD_KERNEL_STREAM : PJZ_FDTD_KERNEL_REFERENCE;
-  if (kernel == PJZ_FDTD_KERNEL_STREAM)
-    e = pjz::run_stream(g, *p, absorption_mask, src, win, ws, L, nsteps, s);
-  else
-    e = pjz::run_reference(g, *p, absorption_mask, src, win, ws, L, nsteps, s);
+  const int nsteps = pjz::steps_of(*p);
+  int cur = 0;
+  if (L.nck > 0) {
+    // Checkpoint the state before steps 0, S, 2S, ... for pjz_fdtd_backward.
+    for (int k = 0; k < L.nck && e == hipSuccess; ++k) {
+      const int b = k * L.S, end = b + L.S < nsteps ? b + L.S : nsteps;
+      e = save_state(g, L, ws, cur, k, s);
+      if (e == hipSuccess) e = run_steps(g, *p, absorption_mask, src, win, ws, L, b, end, &cur,
+                                         nullptr, s);
+    }
+  } else {
+    e = run_steps(g, *p, absorption_mask, src, win, ws, L, 0, nsteps, &cur, nullptr, s);
+  }
   if (e != hipSuccess) return hip_fail(e, "step launch");
+  return PJZ_FDTD_OK;
+}
+
+int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
+                      const float* absorption_mask, const float* pml_kappa,
+                      const float* pml_sigma, const float* pml_alpha,
+                      const float* source_field, const float* source_waveform,
+                      const float* grad_out, float* grad_eps, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+  int rc = check(p);
+  if (rc) return rc;
+  if (p->ckpt_every <= 0)
+    return fail(PJZ_FDTD_EINVAL, "backward needs ckpt_every > 0 (and a forward call with it)");
+  if (!epsilon || !absorption_mask || !pml_kappa || !pml_sigma || !pml_alpha || !source_field ||
+      !source_waveform || !grad_out || !grad_eps || !workspace)
+    return fail(PJZ_FDTD_EINVAL, "a required pointer argument is NULL");
+  pjz::Geom g = geom_of(*p);
+  pjz::Layout L = pjz::make_layout(g, *p);
+  if (workspace_bytes < L.total)
+    return fail(PJZ_FDTD_ESPACE, "workspace of %zu bytes, %zu needed", workspace_bytes, L.total);
+  if (((uintptr_t)workspace & 255) != 0)
+    return fail(PJZ_FDTD_EINVAL, "workspace must be 256-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  const long long f3 = 3 * g.N;
+  const size_t P = (size_t)g.X * g.Y * (size_t)(g.NP > 0 ? g.NP : 1);
+  const int nsteps = pjz::steps_of(*p);
+  hipError_t e = hipSuccess;
+  if (nsteps == 0) {
+    e = pjz::launch_zero(grad_eps, 3 * (size_t)p->ex * p->ey * p->ez, s);
+    return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "zero grad");
+  }
+  e = pjz::launch_prepare(g, *p, epsilon, absorption_mask, pml_kappa, pml_sigma, pml_alpha,
+                          (float*)(ws + L.cb), (float*)(ws + L.zt), s);
+  // Clear lE, lH, lu, lpE, A, B (contiguous from L.lE to the end).
+  if (e == hipSuccess) e = pjz::launch_zero((float*)(ws + L.lE), (L.total - L.lE) / 4, s);
+  if (e != hipSuccess) return hip_fail(e, "backward set-up");
+  pjz::Src src{p->src_axis, p->src_pos, p->src_quad, source_field, source_waveform};
+  pjz::OutWin nowin{p->x0, p->y0, p->z0, p->ex, p->ey, p->ez, nullptr};
+  float* seg = (float*)(ws + L.seg);
+  float* lE = (float*)(ws + L.lE);
+  const long long gstride = 3LL * p->ex * p->ey * p->ez;
+  (void)P;
+  for (int k = L.nck - 1; k >= 0 && e == hipSuccess; --k) {
+    const int b = k * L.S, end = b + L.S < nsteps ? b + L.S : nsteps;
+    // Recompute the segment from its checkpoint, capturing E before each step.
+    int cur = 0;
+    e = restore_state(g, L, ws, k, s);
+    if (e == hipSuccess) e = pjz::launch_copy(seg, (const float*)(ws + L.E0), f3, s);
+    if (e == hipSuccess)
+      e = run_steps(g, *p, absorption_mask, src, nowin, ws, L, b, end, &cur, seg, s);
+    for (int n = end - 1; n >= b && e == hipSuccess; --n) {
+      if (n >= p->out_start && n < p->out_stop && (n - p->out_start) % p->out_step == 0)
+        e = pjz::launch_inject(g, *p, lE, grad_out + gstride * ((n - p->out_start) / p->out_step),
+                               s);
+      if (e == hipSuccess)
+        e = pjz::launch_adjoint_step(g, *p, absorption_mask, ws, L, seg + (n - b) * f3,
+                                     seg + (n - b + 1) * f3, s);
+    }
+  }
+  if (e == hipSuccess)
+    e = pjz::launch_grad_eps(g, *p, epsilon, absorption_mask, ws, L, grad_eps, s);
+  if (e != hipSuccess) return hip_fail(e, "backward launch");
   return PJZ_FDTD_OK;
 }
 

@@ -45,14 +45,32 @@ struct OutWin {
 
 // Byte offsets of the workspace segments (each 256-byte aligned).
 struct Layout {
-  size_t E0, H0, E1, H1, cb, psi, zt, total;
+  size_t E0, H0, E1, H1, cb, psi, zt;
+  // Gradient support (only when ckpt_every > 0):
+  size_t ckpt;        // nck checkpoints of ckpt_floats each
+  size_t seg;         // (S + 1) x 3N: E before every step of one segment
+  size_t lE, lH;      // adjoint E and H (3N each)
+  size_t lu, lpE;     // psi_H and psi_E cotangents (2P each)
+  size_t accA, accB;  // gradient accumulators (3N each)
+  size_t total;
+  int S, nck;
+  size_t ckpt_floats;  // E (3N) + H (3N) + psi_H pair (2P) + psi_E pair (2P)
 };
 
 inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-inline Layout make_layout(const Geom& g) {
+// Number of FDTD steps a call runs: the last snapshot step + 1.
+inline int steps_of(const pjz_fdtd_problem& p) {
+  if (p.out_stop <= p.out_start) return 0;
+  int nn = (p.out_stop - p.out_start + p.out_step - 1) / p.out_step;
+  return p.out_start + (nn - 1) * p.out_step + 1;
+}
+
+inline Layout make_layout(const Geom& g, const pjz_fdtd_problem& p) {
   Layout L;
-  size_t f3 = align256(sizeof(float) * 3 * (size_t)g.N);
+  const size_t P = (size_t)g.X * g.Y * (size_t)(g.NP > 0 ? g.NP : 1);
+  const size_t f3 = align256(sizeof(float) * 3 * (size_t)g.N);
+  const size_t p2 = align256(sizeof(float) * 2 * P);
   size_t off = 0;
   L.E0 = off; off += f3;
   L.H0 = off; off += f3;
@@ -60,11 +78,40 @@ inline Layout make_layout(const Geom& g) {
   L.H1 = off; off += f3;
   L.cb = off; off += f3;
   L.psi = off;
-  off += align256(sizeof(float) * PSI_N * (size_t)g.X * g.Y * (size_t)(g.NP > 0 ? g.NP : 1));
+  off += align256(sizeof(float) * PSI_N * P);
   L.zt = off; off += align256(sizeof(float) * ZROWS * (size_t)g.Z);
+  L.S = 0;
+  L.nck = 0;
+  L.ckpt_floats = 6 * (size_t)g.N + 4 * P;
+  L.ckpt = L.seg = L.lE = L.lH = L.lu = L.lpE = L.accA = L.accB = off;
+  const int nsteps = steps_of(p);
+  if (p.ckpt_every > 0 && nsteps > 0) {
+    L.S = p.ckpt_every < nsteps ? p.ckpt_every : nsteps;
+    L.nck = (nsteps + L.S - 1) / L.S;
+    L.ckpt = off; off += align256(sizeof(float) * L.ckpt_floats) * (size_t)L.nck;
+    L.seg = off; off += f3 * (size_t)(L.S + 1);
+    L.lE = off; off += f3;
+    L.lH = off; off += f3;
+    L.lu = off; off += p2;
+    L.lpE = off; off += p2;
+    L.accA = off; off += f3;
+    L.accB = off; off += f3;
+  }
   L.total = off;
   return L;
 }
+
+inline size_t ckpt_stride(const Layout& L) { return align256(sizeof(float) * L.ckpt_floats); }
+
+// A run of steps [begin, end).  *cur names the ping-pong buffer set that holds
+// the state before the run and is updated to the set that holds it after (the
+// reference stepper works in place on set 0).  If capture is set, the E of
+// the whole domain after step n is also written to capture + (n-begin+1)*3N.
+struct StepRange {
+  int begin, end;
+  int* cur;
+  float* capture;
+};
 
 // Launchers implemented in the kernel translation units.
 hipError_t launch_prepare(const Geom& g, const pjz_fdtd_problem& p, const float* eps,
@@ -75,12 +122,31 @@ hipError_t launch_prepare(const Geom& g, const pjz_fdtd_problem& p, const float*
 // inside a captured graph).
 hipError_t launch_zero(float* ptr, size_t n, hipStream_t s);
 
+// Copies n floats with a copy kernel.
+hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t s);
+
 hipError_t run_reference(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                          const Src& src, const OutWin& win, char* ws, const Layout& L,
-                         int nsteps, hipStream_t s);
+                         const StepRange& r, hipStream_t s);
 
 hipError_t run_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                       const Src& src, const OutWin& win, char* ws, const Layout& L,
-                      int nsteps, hipStream_t s);
+                      const StepRange& r, hipStream_t s);
+
+// Reverse of step n: lE, lH, lu, lpE hold the cotangents after step n on
+// entry and before it on exit; A, B accumulate lE' E^n and lE' E^(n+1).
+hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                               char* ws, const Layout& L, const float* En, const float* En1,
+                               hipStream_t s);
+
+// lE[window] += g (one snapshot cotangent, (3, ex, ey, ez)).
+hipError_t launch_inject(const Geom& g, const pjz_fdtd_problem& p, float* lE, const float* g_out,
+                         hipStream_t s);
+
+// grad_eps (3, ex, ey, ez) = sum over the cells each window cell fills of
+// A dca/deps + (B - ca A)/cb dcb/deps.
+hipError_t launch_grad_eps(const Geom& g, const pjz_fdtd_problem& p, const float* eps,
+                           const float* mask, const char* ws, const Layout& L, float* grad_eps,
+                           hipStream_t s);
 
 }  // namespace pjz

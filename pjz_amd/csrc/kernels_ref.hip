@@ -120,7 +120,7 @@ __global__ void k_ref_h(const float* __restrict__ E, float* __restrict__ H,
 __global__ void k_ref_e(float* __restrict__ E, const float* __restrict__ H,
                         float* __restrict__ psi, const float* __restrict__ zt,
                         const float* __restrict__ cb, const float* __restrict__ mask, Geom g,
-                        Src src, int n, OutWin win, int out_idx) {
+                        Src src, int n, OutWin win, int out_idx, float* __restrict__ Ecap) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= g.N) return;
   const long long N = g.N, YZ = (long long)g.Y * g.Z;
@@ -190,6 +190,7 @@ __global__ void k_ref_e(float* __restrict__ E, const float* __restrict__ H,
     float a = 1.0f - s * b;
     float e = a * E[c * N + idx] + b * c3[c];
     E[c * N + idx] = e;
+    if (Ecap) Ecap[c * N + idx] = e;
     if (in_win) {
       long long o = ((((long long)out_idx * 3 + c) * win.ex + (i - win.x0)) * win.ey +
                      (j - win.y0)) * win.ez + (k - win.z0);
@@ -200,21 +201,36 @@ __global__ void k_ref_e(float* __restrict__ E, const float* __restrict__ H,
 
 hipError_t run_reference(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                          const Src& src, const OutWin& win, char* ws, const Layout& L,
-                         int nsteps, hipStream_t s) {
+                         const StepRange& r, hipStream_t s) {
   float* E = (float*)(ws + L.E0);
   float* H = (float*)(ws + L.H0);
   float* cb = (float*)(ws + L.cb);
   float* psi = (float*)(ws + L.psi);
   const float* zt = (const float*)(ws + L.zt);
   unsigned nb = blocks_for(g.N, 256);
-  for (int n = 0; n < nsteps; ++n) {
+  for (int n = r.begin; n < r.end; ++n) {
     int out_idx = -1;
-    if (n >= p.out_start && n < p.out_stop && (n - p.out_start) % p.out_step == 0)
+    if (win.out && n >= p.out_start && n < p.out_stop && (n - p.out_start) % p.out_step == 0)
       out_idx = (n - p.out_start) / p.out_step;
+    float* cap = r.capture ? r.capture + (long long)(n - r.begin + 1) * 3 * g.N : nullptr;
     hipLaunchKernelGGL(k_ref_h, dim3(nb), dim3(256), 0, s, E, H, psi, zt, g, p.dt);
     hipLaunchKernelGGL(k_ref_e, dim3(nb), dim3(256), 0, s, E, H, psi, zt, cb, mask, g, src, n,
-                       win, out_idx);
+                       win, out_idx, cap);
   }
+  *r.cur = 0;
+  return hipGetLastError();
+}
+
+__global__ void k_copy(float* __restrict__ dst, const float* __restrict__ src, long long n) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t s) {
+  unsigned nb = blocks_for((long long)n, 256);
+  if (nb > 65536u) nb = 65536u;
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, s, dst, src, (long long)n);
   return hipGetLastError();
 }
 

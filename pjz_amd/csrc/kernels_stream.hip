@@ -36,6 +36,7 @@ struct StreamArgs {
   float dt;
   int n;        // time step
   int out_idx;  // snapshot slot or -1
+  float* Ecap;  // if set, the new E of the whole domain is also written here
   int LX;       // x planes per workgroup
   int nty;      // number of y tiles
   int ntiles;   // total tiles
@@ -351,6 +352,10 @@ k_stream(StreamArgs a) {
       }
 #pragma unroll
       for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.Eout + k * N + o, en[k], zb, Z);
+      if (a.Ecap) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.Ecap + k * N + o, en[k], zb, Z);
+      }
       if (a.out_idx >= 0 && x >= a.win.x0 && x < a.win.x0 + a.win.ex && y >= a.win.y0 &&
           y < a.win.y0 + a.win.ey) {
 #pragma unroll
@@ -378,17 +383,21 @@ k_stream(StreamArgs a) {
 }
 
 template <int CPL, int WY, bool vec, int OCC>
-static hipError_t launch_stream(StreamArgs& a, int nsteps, const pjz_fdtd_problem& p,
+static hipError_t launch_stream(StreamArgs& a, const StepRange& r, const pjz_fdtd_problem& p,
                                 float* E0, float* H0, float* E1, float* H1, float* psi,
                                 long long pstride, hipStream_t s) {
   a.nty = (a.g.Y + WY - 1) / WY;
   a.ntiles = a.nty * ((a.g.X + a.LX - 1) / a.LX);
-  for (int n = 0; n < nsteps; ++n) {
+  const long long f3 = 3 * a.g.N;
+  for (int n = r.begin; n < r.end; ++n) {
     a.n = n;
     a.out_idx = -1;
-    if (n >= p.out_start && n < p.out_stop && (n - p.out_start) % p.out_step == 0)
+    if (a.win.out && n >= p.out_start && n < p.out_stop && (n - p.out_start) % p.out_step == 0)
       a.out_idx = (n - p.out_start) / p.out_step;
-    const bool even = (n & 1) == 0;
+    a.Ecap = r.capture ? r.capture + (long long)(n - r.begin + 1) * f3 : nullptr;
+    // The state is read from buffer set *cur and written to the other one.
+    const bool even = *r.cur == 0;
+    *r.cur = even ? 1 : 0;
     a.Ein = even ? E0 : E1;
     a.Hin = even ? H0 : H1;
     a.Eout = even ? E1 : E0;
@@ -403,7 +412,7 @@ static hipError_t launch_stream(StreamArgs& a, int nsteps, const pjz_fdtd_proble
 
 hipError_t run_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                       const Src& src, const OutWin& win, char* ws, const Layout& L,
-                      int nsteps, hipStream_t s) {
+                      const StepRange& r, hipStream_t s) {
   if (g.Z > 256) return hipErrorInvalidValue;
   StreamArgs a;
   float* psi = (float*)(ws + L.psi);
@@ -416,17 +425,19 @@ hipError_t run_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mas
   a.src = src;
   a.win = win;
   a.dt = p.dt;
-  // Default tiling: WY = 8 rows per workgroup and x chunks sized so that the
-  // grid has about one workgroup per CU (measured best at 512x512x128: the
-  // 9-wave workgroups then run in a single round, with 1/LX ghost planes).
-  const int wy_default = 8;
+  a.Ecap = nullptr;
+  // Rows per workgroup: 8 at CPL = 2 (measured best at 512x512x128), else 4.
+  const bool cpl2 = g.Z > 64 && g.Z <= 128 && g.Z % 2 == 0;
+  int wy = cpl2 ? 8 : 4;
+  if (cpl2 && p.tile_y > 0) wy = p.tile_y;
+  // x chunks sized so the grid has about one workgroup per CU: the 9-wave
+  // workgroups then run in a single round, with 1/LX ghost planes.
   if (p.tile_x > 0) {
     a.LX = p.tile_x;
   } else {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int wy = p.tile_y > 0 ? p.tile_y : wy_default;
     const int nty = (g.Y + wy - 1) / wy;
     const int ntx = std::max(1, (cus + nty - 1) / nty);
     a.LX = std::max(8, (g.X + ntx - 1) / ntx);
@@ -436,11 +447,10 @@ hipError_t run_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mas
   float* E1 = (float*)(ws + L.E1);
   float* H1 = (float*)(ws + L.H1);
 #define PJZ_LAUNCH(CPL, WY, VEC, OCC) \
-  return launch_stream<CPL, WY, VEC, OCC>(a, nsteps, p, E0, H0, E1, H1, psi, pstride, s)
+  return launch_stream<CPL, WY, VEC, OCC>(a, r, p, E0, H0, E1, H1, psi, pstride, s)
   if (g.Z <= 64) PJZ_LAUNCH(1, 4, true, 1);
   if (g.Z <= 128) {
-    if (g.Z % 2 != 0) PJZ_LAUNCH(2, 4, false, 1);
-    const int wy = p.tile_y > 0 ? p.tile_y : 8;
+    if (!cpl2) PJZ_LAUNCH(2, 4, false, 1);
     if (wy == 2) PJZ_LAUNCH(2, 2, true, 1);
     if (wy == 4) PJZ_LAUNCH(2, 4, true, 1);
     if (wy == 12) PJZ_LAUNCH(2, 12, true, 1);

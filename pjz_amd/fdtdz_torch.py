@@ -5,10 +5,13 @@ exactly the keywords of :func:`fdtdz` below.  This module keeps that keyword
 set, the array shapes pjz passes and the ``(nn, 3, xx, yy, zz)`` return shape
 (``_field.py:189-191,276-279``), and runs the simulation through
 ``libpjzfdtd.so`` on the current HIP stream.  Arrays may be torch tensors or
-numpy arrays; the result is a float32 torch tensor on the GPU.
+numpy arrays; the result is a float32 torch tensor on the GPU.  When
+``epsilon`` requires grad, the call is a ``torch.autograd.Function`` whose
+backward runs the discrete adjoint kernels (``pjz_fdtd_backward``).
 """
 
 import ctypes
+import math
 from collections.abc import Mapping
 
 import numpy as np
@@ -18,13 +21,15 @@ from . import _lib
 
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "reference": _lib.KERNEL_REFERENCE,
             "stream": _lib.KERNEL_STREAM}
+_LAUNCH_KEYS = {"kernel", "tile_x", "tile_y", "waves_per_simd", "ckpt_every"}
 
 
 def _launch_options(launch_params):
   """Parses ``launch_params``: None, a mapping, or a tuple of (key, value).
 
   Keys: ``kernel`` ("auto" | "reference" | "stream"), ``tile_x``, ``tile_y``,
-  ``waves_per_simd``.
+  ``waves_per_simd`` (stream kernel tiling), ``ckpt_every`` (steps between
+  checkpoints kept for the gradient; default about sqrt(2 * steps)).
   """
   if launch_params is None:
     return {}
@@ -36,7 +41,7 @@ def _launch_options(launch_params):
     except (TypeError, ValueError):
       raise ValueError(f"launch_params must be None, a mapping or (key, value) pairs, "
                        f"got {launch_params!r}") from None
-  unknown = set(opts) - {"kernel", "tile_x", "tile_y", "waves_per_simd"}
+  unknown = set(opts) - _LAUNCH_KEYS
   if unknown:
     raise ValueError(f"unknown launch_params keys {sorted(unknown)}")
   if "kernel" in opts and opts["kernel"] not in _KERNELS:
@@ -102,7 +107,20 @@ def make_problem(epsilon_shape, dt, source_shape, source_position, mask_shape,
   p.tile_x = int(opts.get("tile_x", 0))
   p.tile_y = int(opts.get("tile_y", 0))
   p.waves_per_simd = int(opts.get("waves_per_simd", 0))
+  p.ckpt_every = int(opts.get("ckpt_every", 0))
   return p
+
+
+def steps_of(prob):
+  """FDTD steps a call runs: the last snapshot step + 1 (0 if none)."""
+  r = range(prob.out_start, prob.out_stop, prob.out_step)
+  return r[-1] + 1 if len(r) else 0
+
+
+def default_ckpt_every(nsteps):
+  """Checkpoint spacing that balances checkpoints (~24 B/cell each) against
+  the per-step E of one segment (12 B/cell each): about sqrt(2 * steps)."""
+  return max(1, min(nsteps, int(round(math.sqrt(2.0 * max(nsteps, 1))))))
 
 
 def _device_of(arrays):
@@ -126,6 +144,66 @@ def _ptr(t):
   return ctypes.c_void_p(t.data_ptr())
 
 
+class _Workspace:
+  """Device scratch for one call, 256-byte aligned."""
+
+  def __init__(self, nbytes, device):
+    self.nbytes = nbytes
+    self.buf = torch.empty(nbytes + 256, dtype=torch.uint8, device=device)
+    self.ptr = ctypes.c_void_p((self.buf.data_ptr() + 255) & ~255)
+
+
+def _forward(prob, dev, device):
+  """Runs pjz_fdtd_forward; returns (out, workspace)."""
+  nn = _lib.num_outputs(prob)
+  out = torch.empty((nn, 3, prob.ex, prob.ey, prob.ez), dtype=torch.float32, device=device)
+  ws = None
+  if nn == 0:
+    return out, ws
+  ws = _Workspace(_lib.workspace_bytes(prob), device)
+  with torch.cuda.device(device):
+    stream = torch.cuda.current_stream(device).cuda_stream
+    rc = _lib.load().pjz_fdtd_forward(
+        ctypes.byref(prob), _ptr(dev[0]), _ptr(dev[3]), _ptr(dev[4]), _ptr(dev[5]),
+        _ptr(dev[6]), _ptr(dev[1]), _ptr(dev[2]), _ptr(out), ws.ptr,
+        ctypes.c_size_t(ws.nbytes), ctypes.c_void_p(stream))
+  _lib.check(rc, "pjz_fdtd_forward")
+  # dev and ws were allocated on this stream, so the caching allocator only
+  # reuses their memory for work ordered after the kernels just enqueued.
+  return out, ws
+
+
+def _backward(prob, dev, device, ws, grad_out):
+  grad_eps = torch.empty((3, prob.ex, prob.ey, prob.ez), dtype=torch.float32, device=device)
+  if ws is None:
+    return grad_eps.zero_()
+  g = grad_out.detach().to(device=device, dtype=torch.float32).contiguous()
+  with torch.cuda.device(device):
+    stream = torch.cuda.current_stream(device).cuda_stream
+    rc = _lib.load().pjz_fdtd_backward(
+        ctypes.byref(prob), _ptr(dev[0]), _ptr(dev[3]), _ptr(dev[4]), _ptr(dev[5]),
+        _ptr(dev[6]), _ptr(dev[1]), _ptr(dev[2]), _ptr(g), _ptr(grad_eps), ws.ptr,
+        ctypes.c_size_t(ws.nbytes), ctypes.c_void_p(stream))
+  _lib.check(rc, "pjz_fdtd_backward")
+  return grad_eps
+
+
+class FdtdzFunction(torch.autograd.Function):
+  """fdtdz() with a gradient for ``epsilon`` from the adjoint kernels."""
+
+  @staticmethod
+  def forward(ctx, epsilon, prob, dev, device):
+    out, ws = _forward(prob, dev, device)
+    ctx.prob, ctx.dev, ctx.device, ctx.ws = prob, dev, device, ws
+    return out
+
+  @staticmethod
+  def backward(ctx, grad_out):
+    grad = _backward(ctx.prob, ctx.dev, ctx.device, ctx.ws, grad_out)
+    ctx.ws = None
+    return grad, None, None, None
+
+
 def fdtdz(epsilon, dt, source_field, source_waveform, source_position,
           absorption_mask, pml_kappa, pml_sigma, pml_alpha, pml_widths,
           output_steps, use_reduced_precision, launch_params, offset=(0, 0, 0)):
@@ -133,8 +211,9 @@ def fdtdz(epsilon, dt, source_field, source_waveform, source_position,
 
   Same keywords, shapes and meaning as the call at ``src/pjz/_field.py:254-269``.
   Snapshot i is E after time step ``output_steps[0] + i * output_steps[2]``
-  over the epsilon window at ``offset``.  Raises ValueError for inconsistent
-  shapes or arguments, RuntimeError for GPU failures.
+  over the epsilon window at ``offset``.  Differentiable with respect to
+  ``epsilon`` when it is a tensor that requires grad.  Raises ValueError for
+  inconsistent shapes or arguments, RuntimeError for GPU failures.
   """
   arrays = (epsilon, source_field, source_waveform, absorption_mask,
             pml_kappa, pml_sigma, pml_alpha)
@@ -149,24 +228,15 @@ def fdtdz(epsilon, dt, source_field, source_waveform, source_position,
   prob = make_problem(shapes[0], dt, shapes[1], source_position, shapes[3], shapes[4],
                       pml_widths, output_steps, wshape[0], use_reduced_precision,
                       launch_params, offset)
-  nn = _lib.num_outputs(prob)
+  _lib.num_outputs(prob)  # validates before any device work
   device = _device_of(arrays)
-  lib = _lib.load()
-  out = torch.empty((nn, 3, prob.ex, prob.ey, prob.ez), dtype=torch.float32, device=device)
-  if nn == 0:
-    return out
+  _lib.load()
   dev = [_as_dev(a, device) for a in arrays]
-  nbytes = _lib.workspace_bytes(prob)
-  ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=device)
-  base = ws.data_ptr()
-  aligned = (base + 255) & ~255
-  with torch.cuda.device(device):
-    stream = torch.cuda.current_stream(device).cuda_stream
-    rc = lib.pjz_fdtd_forward(
-        ctypes.byref(prob), _ptr(dev[0]), _ptr(dev[3]), _ptr(dev[4]), _ptr(dev[5]),
-        _ptr(dev[6]), _ptr(dev[1]), _ptr(dev[2]), _ptr(out),
-        ctypes.c_void_p(aligned), ctypes.c_size_t(nbytes), ctypes.c_void_p(stream))
-  _lib.check(rc, "pjz_fdtd_forward")
-  # dev and ws were allocated on this stream, so the caching allocator only
-  # reuses their memory for work ordered after the kernels just enqueued.
-  return out
+  wants_grad = (torch.is_grad_enabled() and isinstance(epsilon, torch.Tensor)
+                and epsilon.requires_grad)
+  if not wants_grad:
+    prob.ckpt_every = 0
+    return _forward(prob, dev, device)[0]
+  if prob.ckpt_every <= 0:
+    prob.ckpt_every = default_ckpt_every(steps_of(prob))
+  return FdtdzFunction.apply(epsilon, prob, dev, device)

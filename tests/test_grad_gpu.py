@@ -1,0 +1,106 @@
+"""GPU gradient of fdtdz() with respect to epsilon (adjoint kernels through
+torch.autograd) against the FP64 discrete adjoint oracle, which is itself
+pinned to torch.autograd through the FP64 stepper and to central differences.
+
+Tolerance: relative L2 error <= 1e-3 of the gradient.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint
+from oracle import helpers as H
+from pjz_amd import fdtdz_torch as F
+from pjz_amd import workloads
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-3
+
+
+def rel_l2(a, b):
+  return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def problem(axis, seed, X=20, Y=18, Z=24, tt=90, widths=(5, 4)):
+  rng = np.random.default_rng(seed)
+  eps = (1 + 3 * rng.random((3, X - 4, Y - 4, Z - 6))).astype(np.float32)
+  sf = {0: rng.standard_normal((2, 1, Y, Z)), 1: rng.standard_normal((2, X, 1, Z)),
+        2: rng.standard_normal((2, 2, X, Y, 1))}[axis].astype(np.float32)
+  pos = {0: 6, 1: 8, 2: 11}[axis]
+  t = np.arange(tt)
+  wave = np.stack([np.sin(0.3 * t), np.cos(0.3 * t)], -1) * np.exp(-((t - 25) / 10.0) ** 2)[:, None]
+  kw = dict(dt=0.5, source_field=sf, source_waveform=wave.astype(np.float32),
+            source_position=pos,
+            absorption_mask=H.absorption_mask(X, Y, 4, 5e-3).astype(np.float32),
+            pml_kappa=(1 + 0.5 * rng.random((Z, 2))).astype(np.float32),
+            pml_sigma=H.pml_sigma(widths, Z, 0.5, 3.0).astype(np.float32),
+            pml_alpha=np.full((Z, 2), 0.05, np.float32), pml_widths=widths,
+            output_steps=(tt - 30, tt, 6), use_reduced_precision=False, launch_params=None,
+            offset=(2, 2, 3))
+  nn = len(range(*kw["output_steps"]))
+  g = rng.standard_normal((nn, 3) + eps.shape[1:]).astype(np.float32)
+  return eps, kw, g
+
+
+def gpu_grad(eps, kw, g, **launch):
+  e = torch.tensor(eps, device="cuda", requires_grad=True)
+  kw = dict(kw, launch_params=launch or None)
+  out = F.fdtdz(e, **kw)
+  (out * torch.as_tensor(g, device="cuda")).sum().backward()
+  torch.cuda.synchronize()
+  return e.grad.cpu().numpy(), out.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("kernel", ["reference", "stream"])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_gradient_matches_oracle(kernel, axis):
+  print(f"case kernel={kernel} axis={axis}", flush=True)
+  eps, kw, g = problem(axis, seed=axis)
+  ref, ref_out = adjoint.grad_epsilon(eps, g_out=g, return_out=True, **kw)
+  got, out = gpu_grad(eps, kw, g, kernel=kernel)
+  assert rel_l2(out, ref_out) <= 1e-4
+  err = rel_l2(got, ref)
+  print(f"grad rel L2 {err:.2e}")
+  assert err <= TOL
+
+
+@pytest.mark.parametrize("ckpt", [1, 7, 89, 1000])
+def test_gradient_independent_of_checkpoint_spacing(ckpt):
+  eps, kw, g = problem(2, seed=5)
+  ref = adjoint.grad_epsilon(eps, g_out=g, **kw)
+  got, _ = gpu_grad(eps, kw, g, ckpt_every=ckpt)
+  assert rel_l2(got, ref) <= TOL
+
+
+def test_gradient_kappa_inf():
+  eps, kw, g = problem(1, seed=6)
+  Z = kw["pml_sigma"].shape[0]
+  kw.update(pml_kappa=np.full((Z, 2), np.inf, np.float32),
+            pml_sigma=np.zeros((Z, 2), np.float32), pml_alpha=np.zeros((Z, 2), np.float32))
+  ref = adjoint.grad_epsilon(eps, g_out=g, **kw)
+  got, _ = gpu_grad(eps, kw, g)
+  assert rel_l2(got, ref) <= TOL
+
+
+def test_gradient_layered_64():
+  """64x64x64 layered stack, 400 steps, loss on the last snapshots."""
+  kw = workloads.layered_64(steps=400)
+  kw["output_steps"] = (370, 400, 10)
+  eps = kw.pop("epsilon")
+  rng = np.random.default_rng(1)
+  nn = len(range(*kw["output_steps"]))
+  g = rng.standard_normal((nn, 3) + eps.shape[1:]).astype(np.float32)
+  ref = adjoint.grad_epsilon(eps, g_out=g, **kw)
+  got, _ = gpu_grad(eps, kw, g)
+  err = rel_l2(got, ref)
+  print(f"layered_64 grad rel L2 {err:.2e}")
+  assert err <= TOL
+
+
+def test_no_grad_path_allocates_no_checkpoints():
+  eps, kw, g = problem(0, seed=7)
+  with torch.no_grad():
+    e = torch.tensor(eps, device="cuda", requires_grad=True)
+    out = F.fdtdz(e, **kw)
+  assert not out.requires_grad
