@@ -54,6 +54,7 @@ extern "C" {
 #define PJZ_FDTD_KERNEL_AUTO 0
 #define PJZ_FDTD_KERNEL_REFERENCE 1 /* two launches per step, one thread/cell */
 #define PJZ_FDTD_KERNEL_STREAM 2    /* one fused launch per step, x-sweep    */
+#define PJZ_FDTD_KERNEL_STREAM2 3   /* two steps per launch (temporal block) */
 
 /* Problem description: the static arguments of fdtdz(...). */
 typedef struct pjz_fdtd_problem {

@@ -28,6 +28,8 @@ int count_outputs(const pjz_fdtd_problem& p) {
   return (p.out_stop - p.out_start + p.out_step - 1) / p.out_step;
 }
 
+pjz::Geom geom_of(const pjz_fdtd_problem& p);
+
 // Checks every static argument; mirrors the shape contract pjz relies on
 // (src/pjz/_field.py:222-269).
 int check(const pjz_fdtd_problem* p) {
@@ -70,10 +72,14 @@ int check(const pjz_fdtd_problem* p) {
       return fail(PJZ_FDTD_EINVAL, "last output step %d needs waveform rows > %d (tt=%d)", last,
                   last, p->tt);
   }
-  if (p->kernel < PJZ_FDTD_KERNEL_AUTO || p->kernel > PJZ_FDTD_KERNEL_STREAM)
+  if (p->kernel < PJZ_FDTD_KERNEL_AUTO || p->kernel > PJZ_FDTD_KERNEL_STREAM2)
     return fail(PJZ_FDTD_EINVAL, "unknown kernel id %d", p->kernel);
   if (p->kernel == PJZ_FDTD_KERNEL_STREAM && p->zz > 256)
     return fail(PJZ_FDTD_EINVAL, "the stream kernel needs zz <= 256, got %d", p->zz);
+  if (p->kernel == PJZ_FDTD_KERNEL_STREAM2 && !pjz::stream2_supported(geom_of(*p)))
+    return fail(PJZ_FDTD_EINVAL,
+                "the two-step stream kernel needs zz <= 64, an even zz <= 128 or a zz <= 256 "
+                "divisible by 4, got %d", p->zz);
   return PJZ_FDTD_OK;
 }
 
@@ -96,8 +102,12 @@ hipError_t run_steps(const pjz::Geom& g, const pjz_fdtd_problem& p, const float*
                      hipStream_t s) {
   int kernel = p.kernel;
   if (kernel == PJZ_FDTD_KERNEL_AUTO)
-    kernel = g.Z <= 256 ? PJZ_FDTD_KERNEL_STREAM : PJZ_FDTD_KERNEL_REFERENCE;
+    kernel = pjz::stream2_supported(g) ? PJZ_FDTD_KERNEL_STREAM2
+             : g.Z <= 256             ? PJZ_FDTD_KERNEL_STREAM
+                                      : PJZ_FDTD_KERNEL_REFERENCE;
   pjz::StepRange r{b, end, cur, capture};
+  if (kernel == PJZ_FDTD_KERNEL_STREAM2)
+    return pjz::run_stream2(g, p, mask, src, win, ws, L, r, s);
   if (kernel == PJZ_FDTD_KERNEL_STREAM) return pjz::run_stream(g, p, mask, src, win, ws, L, r, s);
   return pjz::run_reference(g, p, mask, src, win, ws, L, r, s);
 }
@@ -115,7 +125,7 @@ StateParts state_parts(const pjz::Geom& g, const pjz::Layout& L, char* ws, int c
   float* psi = (float*)(ws + L.psi);
   const long long P = (long long)g.X * g.Y * g.NP;
   return {(float*)(ws + (cur ? L.E1 : L.E0)), (float*)(ws + (cur ? L.H1 : L.H0)),
-          psi + (cur ? pjz::PSI_HX2 : pjz::PSI_HX) * P, psi + pjz::PSI_EX * P};
+          psi + (cur ? pjz::PSI_SET : 0) * P, psi + ((cur ? pjz::PSI_SET : 0) + pjz::PSI_EX) * P};
 }
 
 hipError_t copy_state(const pjz::Geom& g, const pjz::Layout& L, char* ws, int cur, int k,

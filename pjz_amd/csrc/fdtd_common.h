@@ -18,11 +18,11 @@ enum ZRow { IKE = 0, BE = 1, CE = 2, IKH = 3, BH = 4, CH = 5, ZROWS = 8 };
 // psi components, each (xx, yy, np) with np = pml_lo + pml_hi:
 //   PSI_HX: d/dz Ey in the Hx update   PSI_HY: d/dz Ex in the Hy update
 //   PSI_EX: d/dz Hy in the Ex update   PSI_EY: d/dz Hx in the Ey update
-//   PSI_HX2, PSI_HY2: second buffer of the H pair.  The streaming kernel
-//   recomputes H on ghost rows/planes owned by other workgroups, so it reads
-//   psi_H from one buffer and writes the other (like E and H themselves).
-enum PsiComp { PSI_HX = 0, PSI_HY = 1, PSI_EX = 2, PSI_EY = 3, PSI_HX2 = 4, PSI_HY2 = 5,
-               PSI_N = 6 };
+// in two sets of PSI_SET components.  The streaming kernels recompute cells
+// owned by other workgroups (ghost rows and planes), so they read psi from one
+// set and write the other, like E and H; the reference stepper uses set 0 in
+// place.
+enum PsiComp { PSI_HX = 0, PSI_HY = 1, PSI_EX = 2, PSI_EY = 3, PSI_SET = 4, PSI_N = 8 };
 
 struct Geom {
   int X, Y, Z;      // domain
@@ -132,6 +132,13 @@ hipError_t run_reference(const Geom& g, const pjz_fdtd_problem& p, const float* 
 hipError_t run_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                       const Src& src, const OutWin& win, char* ws, const Layout& L,
                       const StepRange& r, hipStream_t s);
+
+// Two steps per launch (temporal blocking of depth 2); an odd last step uses
+// run_stream.  stream2_supported: whether the problem's zz allows it.
+bool stream2_supported(const Geom& g);
+hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                       const Src& src, const OutWin& win, char* ws, const Layout& L,
+                       const StepRange& r, hipStream_t s);
 
 // Reverse of step n: lE, lH, lu, lpE hold the cotangents after step n on
 // entry and before it on exit; A, B accumulate lE' E^n and lE' E^(n+1).

@@ -10,12 +10,11 @@
 //     the new H (for E) through LDS;
 //   - the workgroup sweeps x over a chunk [xa, xb), carrying the new Hy, Hz of
 //     plane x-1 in registers; the plane xa-1 is recomputed (ghost plane).
-// E, H and psi_H ping-pong between two buffers (neighbouring workgroups read
-// the old level on their ghost row/plane while this one writes the new);
-// psi_E is only touched by the owner of each cell and is updated in place.
+// E, H and psi ping-pong between two buffer sets: neighbouring workgroups
+// read the old level on their ghost row/plane while this one writes the new.
 #include <algorithm>
 
-#include "fdtd_common.h"
+#include "stream_common.h"
 
 namespace pjz {
 
@@ -26,7 +25,8 @@ struct StreamArgs {
   float* Hout;
   const float* psiHin;  // PSI_HX, PSI_HY of the old level (HY at +pstride)
   float* psiHout;       // same pair, new level
-  float* psiE;          // PSI_EX, PSI_EY (EY at +pstride), updated in place
+  const float* psiEin;  // PSI_EX, PSI_EY of the old level (EY at +pstride)
+  float* psiEout;       // same pair, new level
   const float* cb;
   const float* zt;
   const float* mask;
@@ -42,75 +42,6 @@ struct StreamArgs {
   int ntiles;   // total tiles
 };
 
-// Loads CPL consecutive z cells of a row; cells at z >= Z read as zero.
-template <int CPL, bool vec>
-__device__ inline void load_col(float (&v)[CPL], const float* __restrict__ p, int zb, int Z,
-                                bool valid) {
-  if (!valid || zb >= Z) {
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) v[c] = 0.0f;
-    return;
-  }
-  if (vec) {
-    if constexpr (CPL == 4) {
-      const float4 t = *reinterpret_cast<const float4*>(p + zb);
-      v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-    } else if constexpr (CPL == 2) {
-      float2 t = *reinterpret_cast<const float2*>(p + zb);
-      v[0] = t.x; v[1] = t.y;
-    } else {
-      v[0] = p[zb];
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) v[c] = (zb + c < Z) ? p[zb + c] : 0.0f;
-  }
-}
-
-template <int CPL, bool vec>
-__device__ inline void store_col(float* __restrict__ p, const float (&v)[CPL], int zb, int Z) {
-  if (zb >= Z) return;
-  if (vec) {
-    if constexpr (CPL == 4) {
-      *reinterpret_cast<float4*>(p + zb) = make_float4(v[0], v[1], v[2], v[3]);
-    } else if constexpr (CPL == 2) {
-      *reinterpret_cast<float2*>(p + zb) = make_float2(v[0], v[1]);
-    } else {
-      p[zb] = v[0];
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < CPL; ++c)
-      if (zb + c < Z) p[zb + c] = v[c];
-  }
-}
-
-// Cell z+1 for each of the lane's cells (0 at and beyond Z).
-template <int CPL>
-__device__ inline void z_next(float (&o)[CPL], const float (&v)[CPL], int zb, int Z) {
-  float up = __shfl_down(v[0], 1, 64);
-#pragma unroll
-  for (int c = 0; c < CPL - 1; ++c) o[c] = v[c + 1];
-  o[CPL - 1] = up;
-#pragma unroll
-  for (int c = 0; c < CPL; ++c)
-    if (zb + c + 1 >= Z) o[c] = 0.0f;
-}
-
-// Cell z-1 for each of the lane's cells (0 below z = 0).
-template <int CPL>
-__device__ inline void z_prev(float (&o)[CPL], const float (&v)[CPL], int lane) {
-  float dn = __shfl_up(v[CPL - 1], 1, 64);
-  o[0] = lane == 0 ? 0.0f : dn;
-#pragma unroll
-  for (int c = 1; c < CPL; ++c) o[c] = v[c - 1];
-}
-
-// Workgroup barrier that waits for this wave's LDS (and scalar) operations
-// only.  __syncthreads() also waits for every outstanding global load and
-// store (vmcnt(0)), which would drain the prefetched plane at every barrier.
-// Rows exchange data only through LDS, so LDS ordering is all that is needed.
-__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // Everything one wave reads for plane x of its row, fetched one plane ahead.
 template <int CPL>
@@ -152,8 +83,8 @@ __device__ inline void fetch_plane(Plane<CPL>& P, const StreamArgs& a, int x, in
     const bool pe = pml[c] && owner_cells;
     P.psH[0][c] = ph ? a.psiHin[pb + pidx[c]] : 0.0f;
     P.psH[1][c] = ph ? a.psiHin[pstride + pb + pidx[c]] : 0.0f;
-    P.psE[0][c] = pe ? a.psiE[pb + pidx[c]] : 0.0f;
-    P.psE[1][c] = pe ? a.psiE[pstride + pb + pidx[c]] : 0.0f;
+    P.psE[0][c] = pe ? a.psiEin[pb + pidx[c]] : 0.0f;
+    P.psE[1][c] = pe ? a.psiEin[pstride + pb + pidx[c]] : 0.0f;
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k)
@@ -307,45 +238,15 @@ k_stream(StreamArgs a) {
         if (pml[c]) {
           float p0 = bE[c] * cur.psE[0][c] + cE[c] * dzHy;
           float p1 = bE[c] * cur.psE[1][c] + cE[c] * dzHx;
-          a.psiE[pb + pidx[c]] = p0;
-          a.psiE[pstride + pb + pidx[c]] = p1;
+          a.psiEout[pb + pidx[c]] = p0;
+          a.psiEout[pstride + pb + pidx[c]] = p1;
           tHy += p0;
           tHx += p1;
         }
         float cx = (hn[2][c] - hdn[1][c]) - tHy;
         float cy = tHx - (hn[2][c] - hzp[c]);
         float cz = (hn[1][c] - hyp[c]) - (hn[0][c] - hdn[0][c]);
-        // Source current J(n).
-        const int z = zb + c;
-        if (a.src.axis == 0) {
-          if (x == a.src.pos || x == a.src.pos - 1) {
-            float wv = a.src.wave[2 * a.n + (x == a.src.pos ? 0 : 1)];
-            if (z < Z) {
-              long long so = (long long)y * Z + z;
-              cy -= a.src.field[so] * wv;
-              cz -= a.src.field[(long long)Y * Z + so] * wv;
-            }
-          }
-        } else if (a.src.axis == 1) {
-          if (y == a.src.pos || y == a.src.pos - 1) {
-            float wv = a.src.wave[2 * a.n + (y == a.src.pos ? 0 : 1)];
-            if (z < Z) {
-              long long so = (long long)x * Z + z;
-              cx -= a.src.field[so] * wv;
-              cz -= a.src.field[(long long)X * Z + so] * wv;
-            }
-          }
-        } else if (z == a.src.pos) {
-          const long long xy = (long long)x * Y + y;
-          float jx = 0.0f, jy = 0.0f;
-          for (int q = 0; q < a.src.quad; ++q) {
-            float wv = a.src.wave[2 * a.n + q];
-            jx += a.src.field[(2 * q + 0) * XY + xy] * wv;
-            jy += a.src.field[(2 * q + 1) * XY + xy] * wv;
-          }
-          cx -= jx;
-          cy -= jy;
-        }
+        add_source(a.src, g, a.n, x, y, zb + c, cx, cy, cz);
         en[0][c] = (1.0f - cur.sig[0] * cur.cb[0][c]) * e0[0][c] + cur.cb[0][c] * cx;
         en[1][c] = (1.0f - cur.sig[1] * cur.cb[1][c]) * e0[1][c] + cur.cb[1][c] * cy;
         en[2][c] = (1.0f - cur.sig[2] * cur.cb[2][c]) * e0[2][c] + cur.cb[2][c] * cz;
@@ -402,8 +303,10 @@ static hipError_t launch_stream(StreamArgs& a, const StepRange& r, const pjz_fdt
     a.Hin = even ? H0 : H1;
     a.Eout = even ? E1 : E0;
     a.Hout = even ? H1 : H0;
-    a.psiHin = psi + (even ? PSI_HX : PSI_HX2) * pstride;
-    a.psiHout = psi + (even ? PSI_HX2 : PSI_HX) * pstride;
+    a.psiHin = psi + (even ? 0 : PSI_SET) * pstride;
+    a.psiHout = psi + (even ? PSI_SET : 0) * pstride;
+    a.psiEin = a.psiHin + PSI_EX * pstride;
+    a.psiEout = a.psiHout + PSI_EX * pstride;
     hipLaunchKernelGGL((k_stream<CPL, WY, vec, OCC>), dim3(a.ntiles), dim3(64 * (WY + 1)), 0, s,
                        a);
   }
@@ -417,7 +320,6 @@ hipError_t run_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mas
   StreamArgs a;
   float* psi = (float*)(ws + L.psi);
   const long long pstride = (long long)g.X * g.Y * g.NP;
-  a.psiE = psi + PSI_EX * pstride;
   a.cb = (const float*)(ws + L.cb);
   a.zt = (const float*)(ws + L.zt);
   a.mask = mask;

@@ -1,0 +1,110 @@
+// Device helpers shared by the streaming steppers (one wavefront per z
+// column, CPL consecutive z cells per lane).
+#pragma once
+
+#include "fdtd_common.h"
+
+namespace pjz {
+
+// Loads CPL consecutive z cells of a row; cells at z >= Z read as zero.
+template <int CPL, bool vec>
+__device__ inline void load_col(float (&v)[CPL], const float* __restrict__ p, int zb, int Z,
+                                bool valid) {
+  if (!valid || zb >= Z) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) v[c] = 0.0f;
+    return;
+  }
+  if (vec) {
+    if constexpr (CPL == 4) {
+      const float4 t = *reinterpret_cast<const float4*>(p + zb);
+      v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else if constexpr (CPL == 2) {
+      float2 t = *reinterpret_cast<const float2*>(p + zb);
+      v[0] = t.x; v[1] = t.y;
+    } else {
+      v[0] = p[zb];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) v[c] = (zb + c < Z) ? p[zb + c] : 0.0f;
+  }
+}
+
+template <int CPL, bool vec>
+__device__ inline void store_col(float* __restrict__ p, const float (&v)[CPL], int zb, int Z) {
+  if (zb >= Z) return;
+  if (vec) {
+    if constexpr (CPL == 4) {
+      *reinterpret_cast<float4*>(p + zb) = make_float4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (CPL == 2) {
+      *reinterpret_cast<float2*>(p + zb) = make_float2(v[0], v[1]);
+    } else {
+      p[zb] = v[0];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+      if (zb + c < Z) p[zb + c] = v[c];
+  }
+}
+
+// Cell z+1 for each of the lane's cells (0 at and beyond Z).
+template <int CPL>
+__device__ inline void z_next(float (&o)[CPL], const float (&v)[CPL], int zb, int Z) {
+  float up = __shfl_down(v[0], 1, 64);
+#pragma unroll
+  for (int c = 0; c < CPL - 1; ++c) o[c] = v[c + 1];
+  o[CPL - 1] = up;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c)
+    if (zb + c + 1 >= Z) o[c] = 0.0f;
+}
+
+// Cell z-1 for each of the lane's cells (0 below z = 0).
+template <int CPL>
+__device__ inline void z_prev(float (&o)[CPL], const float (&v)[CPL], int lane) {
+  float dn = __shfl_up(v[CPL - 1], 1, 64);
+  o[0] = lane == 0 ? 0.0f : dn;
+#pragma unroll
+  for (int c = 1; c < CPL; ++c) o[c] = v[c - 1];
+}
+
+// Workgroup barrier that waits for this wave's LDS (and scalar) operations
+// only.  __syncthreads() also waits for every outstanding global load and
+// store (vmcnt(0)), which would drain the prefetched plane at every barrier.
+// Rows exchange data only through LDS, so LDS ordering is all that is needed.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Adds -J(n) of the source plane to the curl terms of cell (x, y, z).
+__device__ inline void add_source(const Src& src, const Geom& g, int n, int x, int y, int z,
+                                  float& cx, float& cy, float& cz) {
+  if (z >= g.Z) return;
+  if (src.axis == 0) {
+    if (x == src.pos || x == src.pos - 1) {
+      const float wv = src.wave[2 * n + (x == src.pos ? 0 : 1)];
+      const long long so = (long long)y * g.Z + z;
+      cy -= src.field[so] * wv;
+      cz -= src.field[(long long)g.Y * g.Z + so] * wv;
+    }
+  } else if (src.axis == 1) {
+    if (y == src.pos || y == src.pos - 1) {
+      const float wv = src.wave[2 * n + (y == src.pos ? 0 : 1)];
+      const long long so = (long long)x * g.Z + z;
+      cx -= src.field[so] * wv;
+      cz -= src.field[(long long)g.X * g.Z + so] * wv;
+    }
+  } else if (z == src.pos) {
+    const long long xy = (long long)x * g.Y + y, XY = (long long)g.X * g.Y;
+    float jx = 0.0f, jy = 0.0f;
+    for (int q = 0; q < src.quad; ++q) {
+      const float wv = src.wave[2 * n + q];
+      jx += src.field[(2 * q + 0) * XY + xy] * wv;
+      jy += src.field[(2 * q + 1) * XY + xy] * wv;
+    }
+    cx -= jx;
+    cy -= jy;
+  }
+}
+
+}  // namespace pjz

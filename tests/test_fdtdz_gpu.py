@@ -17,7 +17,7 @@ from pjz_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = ["reference", "stream"]
+KERNELS = ["reference", "stream", "stream2"]
 TOL = 1e-4
 
 
@@ -119,7 +119,7 @@ def test_native_512_short_run_matches_oracle():
 
 def test_time_shift_invariance_is_exact():
   kw = random_problem(0, tt=100, seed=3)
-  shift = 17
+  shift = 18
   w = kw["source_waveform"]
   kw2 = dict(kw, source_waveform=np.concatenate([np.zeros((shift, 2), np.float32), w]),
              output_steps=tuple(s + shift if i < 2 else s
@@ -128,6 +128,13 @@ def test_time_shift_invariance_is_exact():
     a = run_gpu(kw, kernel)
     b = run_gpu(kw2, kernel)
     np.testing.assert_array_equal(a, b)
+  # The two-step kernel pairs steps differently under an odd shift; its last
+  # odd step runs the one-step kernel, so compare to rounding there.
+  kw3 = dict(kw, source_waveform=np.concatenate([np.zeros((1, 2), np.float32), w]),
+             output_steps=tuple(s + 1 if i < 2 else s for i, s in enumerate(kw["output_steps"])))
+  a = run_gpu(kw, "stream2")
+  b = run_gpu(kw3, "stream2")
+  assert rel_l2(b, a) <= 1e-6
 
 
 def test_zero_source_gives_zero_fields():
@@ -145,11 +152,13 @@ def test_empty_output_range():
 
 
 @pytest.mark.parametrize("tile_x", [7, 32])
-@pytest.mark.parametrize("tile_y", [2, 4, 8, 12, 15])
-def test_stream_launch_variants_match_oracle(tile_x, tile_y):
-  kw = random_problem(2, X=40, Y=30, Z=64 + 32, tt=60, seed=11, widths=(8, 8))
+@pytest.mark.parametrize("kernel,tile_y", [("stream", 2), ("stream", 4), ("stream", 8),
+                                           ("stream", 12), ("stream", 15), ("stream2", 4),
+                                           ("stream2", 8), ("stream2", 12), ("stream2", 13)])
+def test_stream_launch_variants_match_oracle(tile_x, kernel, tile_y):
+  kw = random_problem(2, X=40, Y=30, Z=64 + 32, tt=61, seed=11, widths=(8, 8))
   kw["offset"] = (3, 2, 5)
   ref = C.fdtdz(**kw)
-  kw["launch_params"] = dict(kernel="stream", tile_x=tile_x, tile_y=tile_y)
+  kw["launch_params"] = dict(kernel=kernel, tile_x=tile_x, tile_y=tile_y)
   got = F.fdtdz(**kw).cpu().numpy()
   assert rel_l2(got, ref) <= TOL

@@ -52,7 +52,7 @@ def gpu_grad(eps, kw, g, **launch):
   return e.grad.cpu().numpy(), out.detach().cpu().numpy()
 
 
-@pytest.mark.parametrize("kernel", ["reference", "stream"])
+@pytest.mark.parametrize("kernel", ["reference", "stream", "stream2"])
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_gradient_matches_oracle(kernel, axis):
   print(f"case kernel={kernel} axis={axis}", flush=True)
@@ -65,11 +65,12 @@ def test_gradient_matches_oracle(kernel, axis):
   assert err <= TOL
 
 
-@pytest.mark.parametrize("ckpt", [1, 7, 89, 1000])
-def test_gradient_independent_of_checkpoint_spacing(ckpt):
+@pytest.mark.parametrize("kernel", ["stream", "stream2"])
+@pytest.mark.parametrize("ckpt", [1, 7, 88, 1000])
+def test_gradient_independent_of_checkpoint_spacing(ckpt, kernel):
   eps, kw, g = problem(2, seed=5)
   ref = adjoint.grad_epsilon(eps, g_out=g, **kw)
-  got, _ = gpu_grad(eps, kw, g, ckpt_every=ckpt)
+  got, _ = gpu_grad(eps, kw, g, ckpt_every=ckpt, kernel=kernel)
   assert rel_l2(got, ref) <= TOL
 
 
