@@ -10,7 +10,11 @@ average launch duration, HIP events on the kernel's stream), and the FP64
 CPU oracle timed on a bounded sample of the same workload on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                       [--kernel auto|reference|stream] [--no-cpu-baseline]
+                       [--kernel auto|reference|stream|stream2] [--no-cpu-baseline]
+
+Other workloads (one JSON line each): --workload grad_256 (forward +
+adjoint gradient, 256x256x128, 6000 steps) and --workload batch64 (64
+uneven simulations batched against 64 sequential calls).
 """
 
 import argparse
@@ -76,13 +80,99 @@ def measured_traffic(kernel, workload):
     return json.load(f)["traffic_bytes_per_launch"]
 
 
+def bench_grad(args):
+  """Forward + adjoint gradient of the overlap objective on grad_256."""
+  import torch
+  from pjz_amd import fdtdz, workloads
+  steps = args.time_steps or 6000
+  kw, mode, omegas = workloads.grad_256(steps=steps)
+  dev = torch.device("cuda", 0)
+  eps0 = torch.as_tensor(kw.pop("epsilon"), device=dev)
+  for k, v in list(kw.items()):
+    if hasattr(v, "shape") and not isinstance(v, torch.Tensor):
+      kw[k] = torch.as_tensor(v, device=dev)
+  kw["launch_params"] = dict(kernel=args.kernel, ckpt_every=256)
+  cells = workloads.cells(kw)
+  tsteps = workloads.time_steps(kw)
+
+  def run(with_grad):
+    e = eps0.clone().requires_grad_(with_grad)
+    out = fdtdz(e, **kw)
+    loss = workloads.overlap_objective(out, mode, omegas, kw["output_steps"])
+    if with_grad:
+      loss.backward()
+    return loss
+
+  for _ in range(args.warmup):
+    run(True)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    run(False)
+  torch.cuda.synchronize()
+  fwd = (time.perf_counter() - t0) / args.steps
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    run(True)
+  torch.cuda.synchronize()
+  both = (time.perf_counter() - t0) / args.steps
+  line = {"metric": METRIC, "value": round(cells * tsteps / both / 1e9, 3),
+          "unit": "Gcell-updates/s (forward steps per second of forward + gradient)",
+          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+          "ms_per_step": round(both * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+          "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+          "config": {"workload": f"grad_256: 256x256x128 cells, {tsteps} steps, objective "
+                                 f"sum_w |<E(w) at x=220, mode>|^2, dL/deps by the adjoint, "
+                                 f"checkpoint every 256 steps",
+                     "cells": int(cells), "time_steps": int(tsteps),
+                     "forward_s": round(fwd, 4), "forward_plus_gradient_s": round(both, 4),
+                     "gradient_over_forward": round(both / fwd, 3)}}
+  print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+  """64 uneven simulations: one batched call against 64 sequential calls."""
+  import torch
+  from pjz_amd import fdtdz, fdtdz_batch, workloads
+  steps = args.time_steps or 4000
+  calls = workloads.batch64(steps=steps)
+  cells = sum(int(c["epsilon"].shape[1] * c["epsilon"].shape[2] * c["epsilon"].shape[3])
+              for c in calls)
+  for _ in range(args.warmup):
+    fdtdz_batch(calls[:4])
+    for c in calls[:4]:
+      fdtdz(**c)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    fdtdz_batch(calls)
+  torch.cuda.synchronize()
+  tb = (time.perf_counter() - t0) / args.steps
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    for c in calls:
+      fdtdz(**c)
+  torch.cuda.synchronize()
+  ts = (time.perf_counter() - t0) / args.steps
+  line = {"metric": METRIC, "value": round(cells * steps / tb / 1e9, 3),
+          "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+          "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+          "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+          "config": {"workload": f"batch64: 64 seeded uneven simulations (xx, yy in "
+                                 f"128..512, zz in 64, 128), {steps} steps, 4 snapshots",
+                     "cells": cells, "time_steps": steps, "batched_s": round(tb, 4),
+                     "sequential_s": round(ts, 4), "speedup_vs_sequential": round(ts / tb, 3)}}
+  print(json.dumps(line), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
   ap.add_argument("--steps", type=int, default=3)
   ap.add_argument("--warmup", type=int, default=1)
   ap.add_argument("--workload", default="native_512")
-  ap.add_argument("--kernel", default="auto", choices=["auto", "reference", "stream"])
+  ap.add_argument("--kernel", default="auto",
+                  choices=["auto", "reference", "stream", "stream2"])
   ap.add_argument("--time-steps", type=int, default=0,
                   help="override the FDTD step count (profiling runs only)")
   ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -91,6 +181,10 @@ def main():
     print("bench.py measures one GPU: the simulation is a single-GPU kernel and "
           "multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
     sys.exit(2)
+  if args.workload == "grad_256":
+    return bench_grad(args)
+  if args.workload == "batch64":
+    return bench_batch(args)
 
   import numpy as np
   import torch
@@ -125,11 +219,15 @@ def main():
   ms_per_step = wall * 1e3 / args.steps
   value = cells * tsteps * args.steps / wall / 1e9
 
-  kernel_name = "stream" if args.kernel == "auto" else args.kernel
-  launches_per_step = 1  # stepper launches per FDTD step (reference: H+E pair)
+  kernel_name = "stream2" if args.kernel == "auto" else args.kernel
+  # Stepper launches per run: the two-step kernel advances two steps per
+  # launch (plus a one-step launch for an odd last step); the reference
+  # stepper's H + E launch pair counts as one launch of one step.
+  launches = (tsteps + 1) // 2 if kernel_name == "stream2" else tsteps
+  temporal_block = 2 if kernel_name == "stream2" else 1
   bpc = algorithmic_bytes_per_cell(zz, lo, hi)
   bytes_per_launch = cells * bpc
-  avg_launch_s = (sum(gpu_ms) / len(gpu_ms)) / 1e3 / (tsteps * launches_per_step)
+  avg_launch_s = (sum(gpu_ms) / len(gpu_ms)) / 1e3 / launches
   achieved = bytes_per_launch / avg_launch_s / 1e9
 
   line = {
@@ -142,7 +240,7 @@ def main():
                              f"{tsteps} FDTD steps, "
                              f"{len(range(*kw['output_steps']))} snapshots",
                  "cells": int(cells), "time_steps": int(tsteps), "kernel": kernel_name,
-                 "temporal_block": 1, "pml_widths": [int(lo), int(hi)]},
+                 "temporal_block": temporal_block, "pml_widths": [int(lo), int(hi)]},
       "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                    "traffic": measured_traffic(kernel_name, args.workload),

@@ -106,6 +106,24 @@ PJZ_FDTD_API int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilo
                      void* stream);
 
 /*
+ * Batched forward: n independent simulations of uneven size advanced together
+ * (persistent two-step launches over a prefix-summed (simulation, tile) work
+ * queue), each with the inputs and outputs of pjz_fdtd_forward.  Pointer
+ * arguments are HOST arrays of n device pointers.  No counterpart in pjz,
+ * which runs one fdtdz() per port in a Python list comprehension
+ * (src/pjz/_field.py:354-356); this is the batched form of that loop.
+ * Every problem needs zz <= 64 or an even zz <= 128.
+ */
+PJZ_FDTD_API int pjz_fdtd_batch_workspace_bytes(const pjz_fdtd_problem* probs, int n,
+                                                size_t* bytes);
+PJZ_FDTD_API int pjz_fdtd_batch_forward(const pjz_fdtd_problem* probs, int n,
+                           const float* const* epsilon, const float* const* absorption_mask,
+                           const float* const* pml_kappa, const float* const* pml_sigma,
+                           const float* const* pml_alpha, const float* const* source_field,
+                           const float* const* source_waveform, float* const* out,
+                           void* workspace, size_t workspace_bytes, void* stream);
+
+/*
  * Gradient of sum_s <grad_out[s], out[s]> with respect to epsilon, by the
  * discrete adjoint of the update (reverse time, transposed curls, CPML and
  * absorber), segment by segment from the checkpoints the preceding

@@ -7,7 +7,7 @@ inputs and project its snapshots (``src/pjz/_field.py:56-150``).
 
 from .boundaries import (absorption_mask, frequency_components, output_phases,
                          pml_sigma, ramped_sin, sampling_interval, source_amplitude)
-from .fdtdz_torch import fdtdz
+from .fdtdz_torch import fdtdz, fdtdz_batch
 
-__all__ = ["fdtdz", "absorption_mask", "pml_sigma", "ramped_sin", "sampling_interval",
+__all__ = ["fdtdz", "fdtdz_batch", "absorption_mask", "pml_sigma", "ramped_sin", "sampling_interval",
            "output_phases", "frequency_components", "source_amplitude"]

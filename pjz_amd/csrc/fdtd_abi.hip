@@ -5,6 +5,8 @@
 #include <stdarg.h>
 #include <limits.h>
 
+#include <vector>
+
 #include "fdtd_common.h"
 
 namespace {
@@ -225,6 +227,110 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
     e = run_steps(g, *p, absorption_mask, src, win, ws, L, 0, nsteps, &cur, nullptr, s);
   }
   if (e != hipSuccess) return hip_fail(e, "step launch");
+  return PJZ_FDTD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Problem with gradient support switched off (batches are forward only).
+pjz_fdtd_problem forward_only(const pjz_fdtd_problem& p) {
+  pjz_fdtd_problem q = p;
+  q.ckpt_every = 0;
+  return q;
+}
+
+int batch_plan(const pjz_fdtd_problem* probs, int n, std::vector<pjz_fdtd_problem>& qs,
+               std::vector<pjz::Layout>& Ls, std::vector<size_t>& offs, size_t& meta_off,
+               size_t& total) {
+  if (!probs || n < 1) return fail(PJZ_FDTD_EINVAL, "a batch needs at least one problem");
+  qs.resize(n);
+  Ls.resize(n);
+  offs.resize(n);
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    int rc = check(&probs[i]);
+    if (rc) {
+      char msg[sizeof(g_err)];
+      snprintf(msg, sizeof(msg), "%s", g_err);
+      return fail(rc, "batch item %d: %s", i, msg);
+    }
+    qs[i] = forward_only(probs[i]);
+    pjz::Geom g = geom_of(qs[i]);
+    if (!pjz::batch2_supported(g))
+      return fail(PJZ_FDTD_EINVAL, "batch item %d: zz must be <= 64 or even and <= 128, got %d", i,
+                  g.Z);
+    Ls[i] = pjz::make_layout(g, qs[i]);
+    offs[i] = off;
+    off += pjz::align256(Ls[i].total);
+  }
+  meta_off = off;
+  std::vector<pjz::BatchItem> items(n);
+  for (int i = 0; i < n; ++i) items[i] = {geom_of(qs[i]), &qs[i], nullptr, {}, {}, nullptr, &Ls[i]};
+  total = off + pjz::batch2_meta_bytes(items.data(), n);
+  return PJZ_FDTD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pjz_fdtd_batch_workspace_bytes(const pjz_fdtd_problem* probs, int n, size_t* bytes) {
+  std::vector<pjz_fdtd_problem> qs;
+  std::vector<pjz::Layout> Ls;
+  std::vector<size_t> offs;
+  size_t meta_off = 0, total = 0;
+  int rc = batch_plan(probs, n, qs, Ls, offs, meta_off, total);
+  if (rc) return rc;
+  if (!bytes) return fail(PJZ_FDTD_EINVAL, "bytes is NULL");
+  *bytes = total;
+  return PJZ_FDTD_OK;
+}
+
+int pjz_fdtd_batch_forward(const pjz_fdtd_problem* probs, int n, const float* const* epsilon,
+                           const float* const* absorption_mask, const float* const* pml_kappa,
+                           const float* const* pml_sigma, const float* const* pml_alpha,
+                           const float* const* source_field,
+                           const float* const* source_waveform, float* const* out,
+                           void* workspace, size_t workspace_bytes, void* stream) {
+  std::vector<pjz_fdtd_problem> qs;
+  std::vector<pjz::Layout> Ls;
+  std::vector<size_t> offs;
+  size_t meta_off = 0, total = 0;
+  int rc = batch_plan(probs, n, qs, Ls, offs, meta_off, total);
+  if (rc) return rc;
+  if (!epsilon || !absorption_mask || !pml_kappa || !pml_sigma || !pml_alpha || !source_field ||
+      !source_waveform || !out || !workspace)
+    return fail(PJZ_FDTD_EINVAL, "a required pointer array is NULL");
+  if (workspace_bytes < total)
+    return fail(PJZ_FDTD_ESPACE, "workspace of %zu bytes, %zu needed", workspace_bytes, total);
+  if (((uintptr_t)workspace & 255) != 0)
+    return fail(PJZ_FDTD_EINVAL, "workspace must be 256-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  std::vector<pjz::BatchItem> items(n);
+  for (int i = 0; i < n; ++i) {
+    if (!epsilon[i] || !absorption_mask[i] || !pml_kappa[i] || !pml_sigma[i] || !pml_alpha[i] ||
+        !source_field[i] || !source_waveform[i] || !out[i])
+      return fail(PJZ_FDTD_EINVAL, "batch item %d: a required pointer is NULL", i);
+    const pjz_fdtd_problem& q = qs[i];
+    pjz::Geom g = geom_of(q);
+    char* ws = base + offs[i];
+    const pjz::Layout& L = Ls[i];
+    hipError_t e = pjz::launch_prepare(g, q, epsilon[i], absorption_mask[i], pml_kappa[i],
+                                       pml_sigma[i], pml_alpha[i], (float*)(ws + L.cb),
+                                       (float*)(ws + L.zt), s);
+    if (e == hipSuccess) e = pjz::launch_zero((float*)(ws + L.E0), (L.E1 - L.E0) / 4, s);
+    if (e == hipSuccess && g.NP > 0)
+      e = pjz::launch_zero((float*)(ws + L.psi), (L.zt - L.psi) / 4, s);
+    if (e != hipSuccess) return hip_fail(e, "batch set-up");
+    items[i] = {g, &qs[i], absorption_mask[i],
+                pjz::Src{q.src_axis, q.src_pos, q.src_quad, source_field[i], source_waveform[i]},
+                pjz::OutWin{q.x0, q.y0, q.z0, q.ex, q.ey, q.ez, out[i]}, ws, &Ls[i]};
+  }
+  hipError_t e = pjz::run_batch2(items.data(), n, base + meta_off, s);
+  if (e != hipSuccess) return hip_fail(e, "batch launch");
   return PJZ_FDTD_OK;
 }
 

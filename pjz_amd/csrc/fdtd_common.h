@@ -140,6 +140,25 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
                        const Src& src, const OutWin& win, char* ws, const Layout& L,
                        const StepRange& r, hipStream_t s);
 
+// One simulation of a batch: its problem, arrays and workspace.
+struct BatchItem {
+  Geom g;
+  const pjz_fdtd_problem* p;
+  const float* mask;
+  Src src;
+  OutWin win;
+  char* ws;
+  const Layout* L;
+};
+
+// Batched two-step stepping (all steps of every simulation).  The items'
+// workspaces must be prepared (coefficients, cleared state); `meta` is device
+// scratch of batch2_meta_bytes() for argument views, tile tables and queue
+// counters.
+bool batch2_supported(const Geom& g);
+size_t batch2_meta_bytes(const BatchItem* items, int n);
+hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s);
+
 // Reverse of step n: lE, lH, lu, lpE hold the cotangents after step n on
 // entry and before it on exit; A, B accumulate lE' E^n and lE' E^(n+1).
 hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,

@@ -16,34 +16,39 @@
 // snapshots/captures of both levels on owned cells) are written.  E, H and
 // psi ping-pong between buffer sets, since ghost rows/planes read cells other
 // workgroups rewrite in the same launch.
+//
+// Instruction economy (the kernel is issue-bound once two steps share a
+// pass): offsets are 32-bit (the C ABI bounds 3*cells below 2^31) and H sits
+// at a fixed offset from E, so few pointers and no 64-bit index products stay
+// live in SGPRs.
 #include <algorithm>
+#include <vector>
 
 #include "stream_common.h"
 
 namespace pjz {
 
 struct Stream2Args {
-  const float* Ein;
-  const float* Hin;
-  float* Eout;
-  float* Hout;
-  const float* psiIn;  // set of 4 psi components (HX, HY, EX, EY), old level
-  float* psiOut;       // same, new level
+  const float* Ein;    // old level: E, and H at Ein + hoff
+  float* Eout;         // new level: E, and H at Eout + hoff
+  const float* psiIn;  // old level psi: HX, HY, EX, EY at multiples of P
+  float* psiOut;
   const float* cb;
   const float* zt;
   const float* mask;
-  Geom g;
-  Src src;
-  OutWin win;
+  const float* sfield;
+  const float* swave;
+  float* out;  // snapshots (nn, 3, wex, wey, wez) or null
+  float* cap;  // capture of E after step n (whole domain), after n+1 at +3N; or null
+  int X, Y, Z, lo, hi, NP, N, hoff, P;
+  int saxis, spos, squad;
+  int wx0, wy0, wz0, wex, wey, wez;
   float dt;
-  int n;                  // first of the two steps
-  int out1, out2;         // snapshot slots of steps n and n+1, or -1
-  float* cap1;            // capture of E after step n (whole domain), or null
-  float* cap2;            // capture of E after step n+1, or null
+  int n, out1, out2;
   int LX, nty, ntiles;
+  int ostart, ostop, ostep, nsteps;  // output schedule and length (batched runs)
 };
 
-// Per-lane z coefficients.
 template <int CPL>
 struct ZCoef {
   float ikE[CPL], bE[CPL], cE[CPL], ikH[CPL], bH[CPL], cH[CPL];
@@ -51,9 +56,51 @@ struct ZCoef {
   int pidx[CPL];
 };
 
-// H half step of one row at one plane.  e0: E at the plane, e1: E at the
-// next plane, eup: (Ex, Ez) of the row above, hold: H before, psH: psi_H
-// before; writes H and psi_H after.  Zero for rows outside the domain.
+// Adds -J(step) to the curl terms of cell (x, y, zb + c).
+template <int CPL>
+__device__ inline void add_source2(const Stream2Args& a, int step, int x, int y, int zb,
+                                   float (&cx)[CPL], float (&cy)[CPL], float (&cz)[CPL]) {
+  if (a.saxis == 0) {
+    if (x != a.spos && x != a.spos - 1) return;
+    const float wv = a.swave[2 * step + (x == a.spos ? 0 : 1)];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      if (zb + c < a.Z) {
+        const int so = y * a.Z + zb + c;
+        cy[c] -= a.sfield[so] * wv;
+        cz[c] -= a.sfield[a.Y * a.Z + so] * wv;
+      }
+    }
+  } else if (a.saxis == 1) {
+    if (y != a.spos && y != a.spos - 1) return;
+    const float wv = a.swave[2 * step + (y == a.spos ? 0 : 1)];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      if (zb + c < a.Z) {
+        const int so = x * a.Z + zb + c;
+        cx[c] -= a.sfield[so] * wv;
+        cz[c] -= a.sfield[a.X * a.Z + so] * wv;
+      }
+    }
+  } else {
+    const int c = a.spos - zb;
+    if (c < 0 || c >= CPL) return;
+    const int xy = x * a.Y + y, XY = a.X * a.Y;
+    float jx = a.sfield[xy] * a.swave[2 * step];
+    float jy = a.sfield[XY + xy] * a.swave[2 * step];
+    if (a.squad > 1) {
+      jx += a.sfield[2 * XY + xy] * a.swave[2 * step + 1];
+      jy += a.sfield[3 * XY + xy] * a.swave[2 * step + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < CPL; ++k)
+      if (k == c) {
+        cx[k] -= jx;
+        cy[k] -= jy;
+      }
+  }
+}
+
 template <int CPL>
 __device__ inline void h_row(float (&hn)[3][CPL], float (&psHn)[2][CPL], const float (&e0)[3][CPL],
                              const float (&e1)[3][CPL], const float (&eup)[2][CPL],
@@ -66,14 +113,10 @@ __device__ inline void h_row(float (&hn)[3][CPL], float (&psHn)[2][CPL], const f
   for (int c = 0; c < CPL; ++c) {
     const float dzEy = eyz[c] - e0[1][c];
     const float dzEx = exz[c] - e0[0][c];
-    float tEy = zc.ikH[c] * dzEy, tEx = zc.ikH[c] * dzEx;
-    psHn[0][c] = psHn[1][c] = 0.0f;
-    if (zc.pml[c]) {
-      psHn[0][c] = zc.bH[c] * psH[0][c] + zc.cH[c] * dzEy;
-      psHn[1][c] = zc.bH[c] * psH[1][c] + zc.cH[c] * dzEx;
-      tEy += psHn[0][c];
-      tEx += psHn[1][c];
-    }
+    psHn[0][c] = zc.pml[c] ? zc.bH[c] * psH[0][c] + zc.cH[c] * dzEy : 0.0f;
+    psHn[1][c] = zc.pml[c] ? zc.bH[c] * psH[1][c] + zc.cH[c] * dzEx : 0.0f;
+    const float tEy = zc.ikH[c] * dzEy + psHn[0][c];
+    const float tEx = zc.ikH[c] * dzEx + psHn[1][c];
     hn[0][c] = hold[0][c] - dt * ((eup[1][c] - e0[2][c]) - tEy);
     hn[1][c] = hold[1][c] - dt * (tEx - (e1[2][c] - e0[2][c]));
     hn[2][c] = hold[2][c] - dt * ((e1[1][c] - e0[1][c]) - (eup[0][c] - e0[0][c]));
@@ -81,38 +124,35 @@ __device__ inline void h_row(float (&hn)[3][CPL], float (&psHn)[2][CPL], const f
   }
 }
 
-// E half step of one row at one plane.  e: E before, hn: new H at the plane,
-// hdn: new (Hx, Hz) of the row below, hp: new (Hy, Hz) at the previous plane,
-// psE: psi_E before.  Writes E and psi_E after.  Zero for rows outside.
 template <int CPL>
 __device__ inline void e_row(float (&en)[3][CPL], float (&psEn)[2][CPL], const float (&e)[3][CPL],
                              const float (&hn)[3][CPL], const float (&hdn)[2][CPL],
                              const float (&hp)[2][CPL], const float (&cb)[3][CPL],
                              const float (&sig)[3], const float (&psE)[2][CPL],
-                             const ZCoef<CPL>& zc, const Src& src, const Geom& g, int step, int x,
-                             int y, int zb, int lane, bool valid) {
+                             const ZCoef<CPL>& zc, const Stream2Args& a, int step, int x, int y,
+                             int zb, int lane, bool valid) {
   float hxz[CPL], hyz[CPL];
   z_prev<CPL>(hxz, hn[0], lane);
   z_prev<CPL>(hyz, hn[1], lane);
+  float cx[CPL], cy[CPL], cz[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const float dzHy = hn[1][c] - hyz[c];
     const float dzHx = hn[0][c] - hxz[c];
-    float tHy = zc.ikE[c] * dzHy, tHx = zc.ikE[c] * dzHx;
-    psEn[0][c] = psEn[1][c] = 0.0f;
-    if (zc.pml[c]) {
-      psEn[0][c] = zc.bE[c] * psE[0][c] + zc.cE[c] * dzHy;
-      psEn[1][c] = zc.bE[c] * psE[1][c] + zc.cE[c] * dzHx;
-      tHy += psEn[0][c];
-      tHx += psEn[1][c];
-    }
-    float cx = (hn[2][c] - hdn[1][c]) - tHy;
-    float cy = tHx - (hn[2][c] - hp[1][c]);
-    float cz = (hn[1][c] - hp[0][c]) - (hn[0][c] - hdn[0][c]);
-    if (valid) add_source(src, g, step, x, y, zb + c, cx, cy, cz);
-    en[0][c] = (1.0f - sig[0] * cb[0][c]) * e[0][c] + cb[0][c] * cx;
-    en[1][c] = (1.0f - sig[1] * cb[1][c]) * e[1][c] + cb[1][c] * cy;
-    en[2][c] = (1.0f - sig[2] * cb[2][c]) * e[2][c] + cb[2][c] * cz;
+    psEn[0][c] = zc.pml[c] ? zc.bE[c] * psE[0][c] + zc.cE[c] * dzHy : 0.0f;
+    psEn[1][c] = zc.pml[c] ? zc.bE[c] * psE[1][c] + zc.cE[c] * dzHx : 0.0f;
+    const float tHy = zc.ikE[c] * dzHy + psEn[0][c];
+    const float tHx = zc.ikE[c] * dzHx + psEn[1][c];
+    cx[c] = (hn[2][c] - hdn[1][c]) - tHy;
+    cy[c] = tHx - (hn[2][c] - hp[1][c]);
+    cz[c] = (hn[1][c] - hp[0][c]) - (hn[0][c] - hdn[0][c]);
+  }
+  if (valid) add_source2<CPL>(a, step, x, y, zb, cx, cy, cz);
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    en[0][c] = (1.0f - sig[0] * cb[0][c]) * e[0][c] + cb[0][c] * cx[c];
+    en[1][c] = (1.0f - sig[1] * cb[1][c]) * e[1][c] + cb[1][c] * cy[c];
+    en[2][c] = (1.0f - sig[2] * cb[2][c]) * e[2][c] + cb[2][c] * cz[c];
     if (!valid) en[0][c] = en[1][c] = en[2][c] = psEn[0][c] = psEn[1][c] = 0.0f;
   }
 }
@@ -123,93 +163,271 @@ struct Plane2 {
   float e[3][CPL];    // input E at x + 1
   float h[3][CPL];    // input H at x
   float cb[3][CPL];   // cb at x (rows with a level-1 E update)
-  float psH[2][CPL];  // input psi_H at x
-  float psE[2][CPL];  // input psi_E at x
+  float ps[4][CPL];   // input psi (HX, HY, EX, EY) at x
   float up[2][CPL];   // input Ex, Ez at (x, y+1) (top row only)
   float sig[3];
 };
 
-template <int CPL, int NW, bool vec>
-__device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y, int w, int zb,
-                              bool active, bool e_rows, const ZCoef<CPL>& zc, long long pstride) {
-  const Geom& g = a.g;
-  const long long N = g.N;
-  const int X = g.X, Y = g.Y, Z = g.Z;
-  const long long o = ((long long)x * Y + y) * Z;
+// State carried from plane x-1 to plane x.
+template <int CPL>
+struct Carry2 {
+  float e0[3][CPL];   // input E at x
+  float e1m[3][CPL];  // level-1 E at x-1
+  float h1m[3][CPL];  // level-1 H at x-1
+  float cbm[3][CPL];  // cb at x-1
+  float psm[4][CPL];  // level-1 psi at x-1
+  float h2p[2][CPL];  // level-2 (Hy, Hz) at x-2
+  float sigm[3];
+};
+
+template <int CPL, bool vec>
+__device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int yc, bool top,
+                              int zb, bool active, bool e_rows, const ZCoef<CPL>& zc) {
+  const int N = a.N, Z = a.Z;
+  const int o = (x * a.Y + yc) * Z;
+  const int o1 = o + a.Y * Z;
+  const bool a1 = active && x + 1 < a.X;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) load_col<CPL, vec>(P.e[k], a.Ein + (k * N + o1), zb, Z, a1);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) load_col<CPL, vec>(P.h[k], a.Ein + (a.hoff + k * N + o), zb, Z, active);
 #pragma unroll
   for (int k = 0; k < 3; ++k)
-    load_col<CPL, vec>(P.e[k], a.Ein + k * N + o + (long long)Y * Z, zb, Z, active && x + 1 < X);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) load_col<CPL, vec>(P.h[k], a.Hin + k * N + o, zb, Z, active);
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    load_col<CPL, vec>(P.cb[k], a.cb + k * N + o, zb, Z, active && e_rows);
-  if (w == NW - 1) {
-    const bool v = active && y + 1 < Y;
-    load_col<CPL, vec>(P.up[0], a.Ein + o + Z, zb, Z, v);
-    load_col<CPL, vec>(P.up[1], a.Ein + 2 * N + o + Z, zb, Z, v);
+    load_col<CPL, vec>(P.cb[k], a.cb + (k * N + o), zb, Z, active && e_rows);
+  if (top) {
+    const bool v = active && yc + 1 < a.Y;
+    load_col<CPL, vec>(P.up[0], a.Ein + (o + Z), zb, Z, v);
+    load_col<CPL, vec>(P.up[1], a.Ein + (2 * N + o + Z), zb, Z, v);
   }
-  const long long pb = ((long long)x * Y + y) * g.NP;
+  const int pb = (x * a.Y + yc) * a.NP;
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const bool ph = zc.pml[c] && active;
-    const long long pi = pb + zc.pidx[c];
-    P.psH[0][c] = ph ? a.psiIn[PSI_HX * pstride + pi] : 0.0f;
-    P.psH[1][c] = ph ? a.psiIn[PSI_HY * pstride + pi] : 0.0f;
-    P.psE[0][c] = ph && e_rows ? a.psiIn[PSI_EX * pstride + pi] : 0.0f;
-    P.psE[1][c] = ph && e_rows ? a.psiIn[PSI_EY * pstride + pi] : 0.0f;
+    const int pi = pb + zc.pidx[c];
+    P.ps[0][c] = ph ? a.psiIn[pi] : 0.0f;
+    P.ps[1][c] = ph ? a.psiIn[a.P + pi] : 0.0f;
+    P.ps[2][c] = ph && e_rows ? a.psiIn[2 * a.P + pi] : 0.0f;
+    P.ps[3][c] = ph && e_rows ? a.psiIn[3 * a.P + pi] : 0.0f;
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k)
-    P.sig[k] = active && e_rows ? a.mask[(k * (long long)X + x) * Y + y] : 0.0f;
+    P.sig[k] = active && e_rows ? a.mask[(k * a.X + x) * a.Y + yc] : 0.0f;
 }
 
-// Stores the E of one row (whole z column) into a snapshot window slot.
 template <int CPL>
-__device__ inline void store_snapshot(const OutWin& win, int slot, int x, int y, int zb,
-                                      const float (&en)[3][CPL]) {
-  if (slot < 0 || x < win.x0 || x >= win.x0 + win.ex || y < win.y0 || y >= win.y0 + win.ey)
-    return;
+__device__ inline void store_snapshot2(const Stream2Args& a, int slot, int x, int y, int zb,
+                                       const float (&en)[3][CPL]) {
+  if (slot < 0 || x < a.wx0 || x >= a.wx0 + a.wex || y < a.wy0 || y >= a.wy0 + a.wey) return;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    float* op = win.out + ((((long long)slot * 3 + k) * win.ex + (x - win.x0)) * win.ey +
-                           (y - win.y0)) * win.ez;
+    float* op = a.out + ((((long long)slot * 3 + k) * a.wex + (x - a.wx0)) * a.wey +
+                         (y - a.wy0)) * (long long)a.wez;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
       const int z = zb + c;
-      if (z >= win.z0 && z < win.z0 + win.ez) op[z - win.z0] = en[k][c];
+      if (z >= a.wz0 && z < a.wz0 + a.wez) op[z - a.wz0] = en[k][c];
     }
   }
 }
 
+template <int CPL, int WY>
+struct Lds2 {
+  float sE1[WY + 3][2][64 * CPL];
+  float sH1[WY + 3][2][64 * CPL];
+  float sE2[2][WY + 3][2][64 * CPL];
+  float sH2[WY + 3][2][64 * CPL];
+};
+
+// One plane of the sweep: level 1 at x, level 2 at x-1.  PAR selects the
+// sE2 slot written this plane (the other slot holds plane x-1).
 template <int CPL, int WY, bool vec>
-__global__ __launch_bounds__(64 * (WY + 3)) void k_stream2(Stream2Args a) {
+__device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, Carry2<CPL>& s,
+                                       const Plane2<CPL>& cur, Plane2<CPL>& nxt,
+                                       const ZCoef<CPL>& zc, int x, int xa, int xb, int w,
+                                       int y, int yc, int lane, int zb, bool row_valid,
+                                       int PAR) {
   constexpr int NW = WY + 3;
-  const Geom g = a.g;
-  const int X = g.X, Y = g.Y, Z = g.Z;
-  const long long N = g.N;
-  const float dt = a.dt;
+  const bool e_rows = w >= 1;
+  const bool h2_rows = w >= 1 && w <= WY + 1;
+  const bool own = w >= 2 && w <= WY + 1;
+  const bool l1 = x < a.X;
+  const bool l2 = x >= xa && x >= 1;
+  const bool act = row_valid && l1;
+  const int Z = a.Z, N = a.N;
+  if (x + 1 <= xb)
+    fetch2<CPL, vec>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, e_rows, zc);
+
+  // ---- A: level-1 H at plane x
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    L.sE1[w][0][zb + c] = s.e0[0][c];
+    L.sE1[w][1][zb + c] = s.e0[2][c];
+  }
+  lds_barrier();
+  float eup[2][CPL];
+  if (w < NW - 1) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      eup[0][c] = L.sE1[w + 1][0][zb + c];
+      eup[1][c] = L.sE1[w + 1][1][zb + c];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      eup[0][c] = cur.up[0][c];
+      eup[1][c] = cur.up[1][c];
+    }
+  }
+  float hn1[3][CPL], psH1[2][CPL];
+  {
+    float psH[2][CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      psH[0][c] = cur.ps[0][c];
+      psH[1][c] = cur.ps[1][c];
+    }
+    h_row<CPL>(hn1, psH1, s.e0, cur.e, eup, cur.h, psH, zc, zb, Z, a.dt, act);
+  }
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    L.sH1[w][0][zb + c] = hn1[0][c];
+    L.sH1[w][1][zb + c] = hn1[2][c];
+  }
+  lds_barrier();
+
+  // ---- B: level-1 E at plane x, then level-2 H at plane x-1
+  float en1[3][CPL], psE1[2][CPL];
+  if (e_rows) {
+    float hdn[2][CPL], hp[2][CPL], psE[2][CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      hdn[0][c] = L.sH1[w - 1][0][zb + c];
+      hdn[1][c] = L.sH1[w - 1][1][zb + c];
+      hp[0][c] = s.h1m[1][c];
+      hp[1][c] = s.h1m[2][c];
+      psE[0][c] = cur.ps[2][c];
+      psE[1][c] = cur.ps[3][c];
+    }
+    e_row<CPL>(en1, psE1, s.e0, hn1, hdn, hp, cur.cb, cur.sig, psE, zc, a, a.n, x, y, zb, lane,
+               act);
+    if (own && act && x >= xa && x < xb) {
+      if (a.out1 >= 0) store_snapshot2<CPL>(a, a.out1, x, y, zb, en1);
+      if (a.cap) {
+        const int o = (x * a.Y + y) * Z;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.cap + (k * N + o), en1[k], zb, Z);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      en1[0][c] = en1[1][c] = en1[2][c] = 0.0f;
+      psE1[0][c] = psE1[1][c] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    L.sE2[PAR][w][0][zb + c] = en1[0][c];
+    L.sE2[PAR][w][1][zb + c] = en1[2][c];
+  }
+  float hn2[3][CPL], psH2[2][CPL];
+  if (h2_rows && l2) {
+    float e2up[2][CPL], psHm[2][CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      e2up[0][c] = L.sE2[PAR ^ 1][w + 1][0][zb + c];
+      e2up[1][c] = L.sE2[PAR ^ 1][w + 1][1][zb + c];
+      psHm[0][c] = s.psm[0][c];
+      psHm[1][c] = s.psm[1][c];
+    }
+    h_row<CPL>(hn2, psH2, s.e1m, en1, e2up, s.h1m, psHm, zc, zb, Z, a.dt, row_valid);
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      hn2[0][c] = hn2[1][c] = hn2[2][c] = 0.0f;
+      psH2[0][c] = psH2[1][c] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    L.sH2[w][0][zb + c] = hn2[0][c];
+    L.sH2[w][1][zb + c] = hn2[2][c];
+  }
+  lds_barrier();
+
+  // ---- C: level-2 E at plane x-1 (the result on owned rows)
+  if (own && l2 && row_valid) {
+    float hdn[2][CPL], psEm[2][CPL], en2[3][CPL], psE2[2][CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      hdn[0][c] = L.sH2[w - 1][0][zb + c];
+      hdn[1][c] = L.sH2[w - 1][1][zb + c];
+      psEm[0][c] = s.psm[2][c];
+      psEm[1][c] = s.psm[3][c];
+    }
+    e_row<CPL>(en2, psE2, s.e1m, hn2, hdn, s.h2p, s.cbm, s.sigm, psEm, zc, a, a.n + 1, x - 1, y,
+               zb, lane, true);
+    if (x - 1 >= xa) {
+      const int o = ((x - 1) * a.Y + y) * Z;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        store_col<CPL, vec>(a.Eout + (k * N + o), en2[k], zb, Z);
+        store_col<CPL, vec>(a.Eout + (a.hoff + k * N + o), hn2[k], zb, Z);
+      }
+      const int pb = ((x - 1) * a.Y + y) * a.NP;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        if (zc.pml[c]) {
+          const int pi = pb + zc.pidx[c];
+          a.psiOut[pi] = psH2[0][c];
+          a.psiOut[a.P + pi] = psH2[1][c];
+          a.psiOut[2 * a.P + pi] = psE2[0][c];
+          a.psiOut[3 * a.P + pi] = psE2[1][c];
+        }
+      }
+      if (a.out2 >= 0) store_snapshot2<CPL>(a, a.out2, x - 1, y, zb, en2);
+      if (a.cap) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.cap + (3 * N + k * N + o), en2[k], zb, Z);
+      }
+    }
+  }
+
+  // ---- carry to plane x+1
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    s.h2p[0][c] = hn2[1][c];
+    s.h2p[1][c] = hn2[2][c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      s.e1m[k][c] = en1[k][c];
+      s.h1m[k][c] = hn1[k][c];
+      s.cbm[k][c] = cur.cb[k][c];
+      s.e0[k][c] = cur.e[k][c];
+    }
+    s.psm[0][c] = psH1[0][c];
+    s.psm[1][c] = psH1[1][c];
+    s.psm[2][c] = psE1[0][c];
+    s.psm[3][c] = psE1[1][c];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) s.sigm[k] = cur.sig[k];
+}
+
+// One tile (WY rows x LX planes) of one simulation for steps a.n, a.n+1.
+template <int CPL, int WY, bool vec>
+__device__ __forceinline__ void stream2_tile(const Stream2Args& a, int tile, Lds2<CPL, WY>& L) {
+  constexpr int NW = WY + 3;
+  const int Z = a.Z;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int zb = lane * CPL;
-
-  const int b = blockIdx.x;
-  int tile = b;
-  if ((a.ntiles & 7) == 0) tile = (b & 7) * (a.ntiles >> 3) + (b >> 3);
   const int ty = tile % a.nty, tx = tile / a.nty;
-  const int y0 = ty * WY;
-  const int y = y0 - 2 + w;
-  const bool row_valid = y >= 0 && y < Y;
-  const bool e_rows = w >= 1;               // rows with a level-1 E update
-  const bool h2_rows = w >= 1 && w <= WY + 1;  // rows with a level-2 H update
-  const bool own = w >= 2 && w <= WY + 1;   // rows whose level-2 E is the result
+  const int y = ty * WY - 2 + w;
+  const bool row_valid = y >= 0 && y < a.Y;
+  const int yc = min(max(y, 0), a.Y - 1);  // address-safe row
   const int xa = tx * a.LX;
-  const int xb = min(xa + a.LX, X);
-
-  __shared__ float sE1[NW][2][64 * CPL];     // input Ex, Ez at plane x
-  __shared__ float sH1[NW][2][64 * CPL];     // level-1 Hx, Hz at plane x
-  __shared__ float sE2[2][NW][2][64 * CPL];  // level-1 Ex, Ez, by plane parity
-  __shared__ float sH2[NW][2][64 * CPL];     // level-2 Hx, Hz at plane x-1
+  const int xb = min(xa + a.LX, a.X);
 
   ZCoef<CPL> zc;
 #pragma unroll
@@ -221,180 +439,82 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2(Stream2Args a) {
     zc.ikH[c] = a.zt[IKH * Z + z];
     zc.bH[c] = a.zt[BH * Z + z];
     zc.cH[c] = a.zt[CH * Z + z];
-    zc.pml[c] = (zb + c < Z) && (zb + c < g.lo || zb + c >= Z - g.hi);
-    zc.pidx[c] = zb + c < g.lo ? zb + c : zb + c - (Z - g.hi) + g.lo;
+    zc.pml[c] = (zb + c < Z) && (zb + c < a.lo || zb + c >= Z - a.hi);
+    zc.pidx[c] = zb + c < a.lo ? zb + c : zb + c - (Z - a.hi) + a.lo;
   }
-  const long long pstride = (long long)X * Y * g.NP;
 
-  // Carried state: input E at x; level-1 E, H, cb, sigma, psi at x-1;
-  // level-2 (Hy, Hz) at x-2.
-  float e0[3][CPL], e1m[3][CPL], h1m[3][CPL], cbm[3][CPL], psm[4][CPL], h2p[2][CPL];
-  float sigm[3] = {0.0f, 0.0f, 0.0f};
+  Carry2<CPL> s;
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) e1m[k][c] = h1m[k][c] = cbm[k][c] = 0.0f;
+    for (int k = 0; k < 3; ++k) s.e1m[k][c] = s.h1m[k][c] = s.cbm[k][c] = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) psm[k][c] = 0.0f;
-    h2p[0][c] = h2p[1][c] = 0.0f;
+    for (int k = 0; k < 4; ++k) s.psm[k][c] = 0.0f;
+    s.h2p[0][c] = s.h2p[1][c] = 0.0f;
   }
+  s.sigm[0] = s.sigm[1] = s.sigm[2] = 0.0f;
   const int xs = max(xa - 2, 0);
   {
-    const long long o = ((long long)xs * Y + y) * Z;
+    const int o = (xs * a.Y + yc) * Z;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) load_col<CPL, vec>(e0[k], a.Ein + k * N + o, zb, Z, row_valid);
+    for (int k = 0; k < 3; ++k) load_col<CPL, vec>(s.e0[k], a.Ein + (k * a.N + o), zb, Z, row_valid);
   }
   Plane2<CPL> cur, nxt;
-  fetch2<CPL, NW, vec>(cur, a, xs, y, w, zb, row_valid, e_rows, zc, pstride);
+  fetch2<CPL, vec>(cur, a, xs, yc, w == NW - 1, zb, row_valid, w >= 1, zc);
   int par = 0;
-
   for (int x = xs; x <= xb; ++x) {
-    const bool l1 = x < X;
-    const bool l2 = x >= xa && x >= 1;
-    const bool st2 = x - 1 >= xa;
-    const bool act = row_valid && l1;
-    if (x + 1 <= xb)
-      fetch2<CPL, NW, vec>(nxt, a, x + 1, y, w, zb, row_valid && x + 1 < X, e_rows, zc, pstride);
-
-    // ---- A: level-1 H at plane x
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      sE1[w][0][zb + c] = e0[0][c];
-      sE1[w][1][zb + c] = e0[2][c];
-    }
-    lds_barrier();
-    float eup[2][CPL];
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      eup[0][c] = w < NW - 1 ? sE1[w + 1][0][zb + c] : cur.up[0][c];
-      eup[1][c] = w < NW - 1 ? sE1[w + 1][1][zb + c] : cur.up[1][c];
-    }
-    float hn1[3][CPL], psH1[2][CPL];
-    h_row<CPL>(hn1, psH1, e0, cur.e, eup, cur.h, cur.psH, zc, zb, Z, dt, act);
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      sH1[w][0][zb + c] = hn1[0][c];
-      sH1[w][1][zb + c] = hn1[2][c];
-    }
-    lds_barrier();
-
-    // ---- B: level-1 E at plane x, then level-2 H at plane x-1
-    float en1[3][CPL], psE1[2][CPL];
-    if (e_rows) {
-      float hdn[2][CPL], hp[2][CPL];
-#pragma unroll
-      for (int c = 0; c < CPL; ++c) {
-        hdn[0][c] = sH1[w - 1][0][zb + c];
-        hdn[1][c] = sH1[w - 1][1][zb + c];
-        hp[0][c] = h1m[1][c];
-        hp[1][c] = h1m[2][c];
-      }
-      e_row<CPL>(en1, psE1, e0, hn1, hdn, hp, cur.cb, cur.sig, cur.psE, zc, a.src, g, a.n, x, y,
-                 zb, lane, act);
-      if (own && act && x >= xa && x < xb) {
-        store_snapshot<CPL>(a.win, a.out1, x, y, zb, en1);
-        if (a.cap1) {
-          const long long o = ((long long)x * Y + y) * Z;
-#pragma unroll
-          for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.cap1 + k * N + o, en1[k], zb, Z);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < CPL; ++c) {
-        en1[0][c] = en1[1][c] = en1[2][c] = 0.0f;
-        psE1[0][c] = psE1[1][c] = 0.0f;
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      sE2[par][w][0][zb + c] = en1[0][c];
-      sE2[par][w][1][zb + c] = en1[2][c];
-    }
-    float hn2[3][CPL], psH2[2][CPL];
-    if (h2_rows && l2) {
-      float e2up[2][CPL], psHm[2][CPL];
-#pragma unroll
-      for (int c = 0; c < CPL; ++c) {
-        e2up[0][c] = sE2[par ^ 1][w + 1][0][zb + c];
-        e2up[1][c] = sE2[par ^ 1][w + 1][1][zb + c];
-        psHm[0][c] = psm[0][c];
-        psHm[1][c] = psm[1][c];
-      }
-      h_row<CPL>(hn2, psH2, e1m, en1, e2up, h1m, psHm, zc, zb, Z, dt, row_valid);
-    } else {
-#pragma unroll
-      for (int c = 0; c < CPL; ++c) {
-        hn2[0][c] = hn2[1][c] = hn2[2][c] = 0.0f;
-        psH2[0][c] = psH2[1][c] = 0.0f;
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      sH2[w][0][zb + c] = hn2[0][c];
-      sH2[w][1][zb + c] = hn2[2][c];
-    }
-    lds_barrier();
-
-    // ---- C: level-2 E at plane x-1 (the result on owned rows)
-    if (own && l2 && row_valid) {
-      float hdn[2][CPL], psEm[2][CPL], en2[3][CPL], psE2[2][CPL];
-#pragma unroll
-      for (int c = 0; c < CPL; ++c) {
-        hdn[0][c] = sH2[w - 1][0][zb + c];
-        hdn[1][c] = sH2[w - 1][1][zb + c];
-        psEm[0][c] = psm[2][c];
-        psEm[1][c] = psm[3][c];
-      }
-      e_row<CPL>(en2, psE2, e1m, hn2, hdn, h2p, cbm, sigm, psEm, zc, a.src, g, a.n + 1, x - 1,
-                 y, zb, lane, true);
-      if (st2) {
-        const long long o = ((long long)(x - 1) * Y + y) * Z;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          store_col<CPL, vec>(a.Eout + k * N + o, en2[k], zb, Z);
-          store_col<CPL, vec>(a.Hout + k * N + o, hn2[k], zb, Z);
-        }
-        const long long pb = ((long long)(x - 1) * Y + y) * g.NP;
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          if (zc.pml[c]) {
-            const long long pi = pb + zc.pidx[c];
-            a.psiOut[PSI_HX * pstride + pi] = psH2[0][c];
-            a.psiOut[PSI_HY * pstride + pi] = psH2[1][c];
-            a.psiOut[PSI_EX * pstride + pi] = psE2[0][c];
-            a.psiOut[PSI_EY * pstride + pi] = psE2[1][c];
-          }
-        }
-        store_snapshot<CPL>(a.win, a.out2, x - 1, y, zb, en2);
-        if (a.cap2) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.cap2 + k * N + o, en2[k], zb, Z);
-        }
-      }
-    }
-
-    // ---- rotate
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      h2p[0][c] = hn2[1][c];
-      h2p[1][c] = hn2[2][c];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        e1m[k][c] = en1[k][c];
-        h1m[k][c] = hn1[k][c];
-        cbm[k][c] = cur.cb[k][c];
-        e0[k][c] = cur.e[k][c];
-      }
-      psm[0][c] = psH1[0][c];
-      psm[1][c] = psH1[1][c];
-      psm[2][c] = psE1[0][c];
-      psm[3][c] = psE1[1][c];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) sigm[k] = cur.sig[k];
+    plane2<CPL, WY, vec>(a, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid, par);
     cur = nxt;
     par ^= 1;
+  }
+}
+
+template <int CPL, int WY, bool vec>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_stream2(Stream2Args a) {
+  __shared__ Lds2<CPL, WY> L;
+  // Consecutive workgroup ids go round-robin over the 8 XCDs; remap so that
+  // tiles adjacent in y (which share ghost rows) run on the same XCD.
+  int tile = blockIdx.x;
+  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
+  stream2_tile<CPL, WY, vec>(a, tile, L);
+}
+
+// Snapshot slot of a step in a simulation's output schedule, or -1.
+__device__ inline int out_slot(const Stream2Args& a, int step) {
+  if (!a.out || step < a.ostart || step >= a.ostop || (step - a.ostart) % a.ostep != 0) return -1;
+  return (step - a.ostart) / a.ostep;
+}
+
+// Batched two-step kernel: persistent workgroups take (simulation, tile) work
+// items from a device counter until the launch's items are exhausted; the
+// simulation of item t is the last s with tile_start[s] <= t (tile counts are
+// prefix-summed over simulations of uneven size).  args holds two views per
+// simulation (which buffer set is old), selected by the parity of the step
+// pair.  Every workgroup exits once the counter passes `total`.
+template <int CPL, int WY, bool vec>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_batch(
+    const Stream2Args* __restrict__ args, const int* __restrict__ tile_start, int nsims,
+    int total, int* __restrict__ counter, int step) {
+  __shared__ Lds2<CPL, WY> L;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (t >= total) break;
+    int lo = 0, hi = nsims - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    Stream2Args a = args[2 * lo + ((step >> 1) & 1)];
+    if (step >= a.nsteps) continue;
+    a.n = step;
+    a.out1 = out_slot(a, step);
+    a.out2 = out_slot(a, step + 1);
+    stream2_tile<CPL, WY, vec>(a, t - tile_start[lo], L);
   }
 }
 
@@ -410,36 +530,80 @@ bool stream2_supported(const Geom& g) {
 
 // Runs steps [r.begin, r.end) two at a time; an odd last step uses the
 // one-step kernel (same buffer sets).
+static int device_cus() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return cus;
+}
+
+// Kernel arguments of one simulation with buffer set `cur` as the old level.
+static Stream2Args make_args2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                             const Src& src, const OutWin& win, char* ws, const Layout& L,
+                             int WY, int LX, int cur) {
+  Stream2Args a;
+  float* psi = (float*)(ws + L.psi);
+  const int P = g.X * g.Y * g.NP;
+  float* E[2] = {(float*)(ws + L.E0), (float*)(ws + L.E1)};
+  a.Ein = E[cur];
+  a.Eout = E[1 - cur];
+  a.psiIn = psi + (cur ? PSI_SET : 0) * (long long)P;
+  a.psiOut = psi + (cur ? 0 : PSI_SET) * (long long)P;
+  a.cb = (const float*)(ws + L.cb);
+  a.zt = (const float*)(ws + L.zt);
+  a.mask = mask;
+  a.sfield = src.field;
+  a.swave = src.wave;
+  a.saxis = src.axis;
+  a.spos = src.pos;
+  a.squad = src.quad;
+  a.out = win.out;
+  a.cap = nullptr;
+  a.wx0 = win.x0;
+  a.wy0 = win.y0;
+  a.wz0 = win.z0;
+  a.wex = win.ex;
+  a.wey = win.ey;
+  a.wez = win.ez;
+  a.X = g.X;
+  a.Y = g.Y;
+  a.Z = g.Z;
+  a.lo = g.lo;
+  a.hi = g.hi;
+  a.NP = g.NP;
+  a.N = (int)g.N;
+  a.P = P;
+  a.hoff = (int)((L.H0 - L.E0) / sizeof(float));
+  a.dt = p.dt;
+  a.n = 0;
+  a.out1 = a.out2 = -1;
+  a.LX = LX;
+  a.nty = (g.Y + WY - 1) / WY;
+  a.ntiles = a.nty * ((g.X + LX - 1) / LX);
+  a.ostart = p.out_start;
+  a.ostop = p.out_stop;
+  a.ostep = p.out_step;
+  a.nsteps = steps_of(p);
+  return a;
+}
+
 hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                        const Src& src, const OutWin& win, char* ws, const Layout& L,
                        const StepRange& r, hipStream_t s) {
   if (!stream2_supported(g)) return hipErrorInvalidValue;
   const int cpl = g.Z <= 64 ? 1 : (g.Z <= 128 ? 2 : 4);
   const int WY = cpl == 4 ? 4 : (p.tile_y > 0 ? p.tile_y : 8);
-  Stream2Args a;
-  float* psi = (float*)(ws + L.psi);
-  const long long pstride = (long long)g.X * g.Y * g.NP;
-  a.cb = (const float*)(ws + L.cb);
-  a.zt = (const float*)(ws + L.zt);
-  a.mask = mask;
-  a.g = g;
-  a.src = src;
-  a.win = win;
-  a.dt = p.dt;
-  if (p.tile_x > 0) {
-    a.LX = p.tile_x;
-  } else {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int LX = p.tile_x;
+  if (LX <= 0) {
+    // x chunks sized so the grid has about one workgroup per CU.
     const int nty = (g.Y + WY - 1) / WY;
-    const int ntx = std::max(1, (cus + nty - 1) / nty);
-    a.LX = std::max(8, (g.X + ntx - 1) / ntx);
+    const int ntx = std::max(1, (device_cus() + nty - 1) / nty);
+    LX = std::max(8, (g.X + ntx - 1) / ntx);
   }
-  a.nty = (g.Y + WY - 1) / WY;
-  a.ntiles = a.nty * ((g.X + a.LX - 1) / a.LX);
+  Stream2Args a = make_args2(g, p, mask, src, win, ws, L, WY, LX, 0);
+  float* psi = (float*)(ws + L.psi);
+  const int P = g.X * g.Y * g.NP;
   float* E[2] = {(float*)(ws + L.E0), (float*)(ws + L.E1)};
-  float* H[2] = {(float*)(ws + L.H0), (float*)(ws + L.H1)};
   const long long f3 = 3 * g.N;
   auto slot = [&](int step) {
     if (!win.out || step < p.out_start || step >= p.out_stop ||
@@ -451,16 +615,13 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
   for (; n + 2 <= r.end; n += 2) {
     const int c = *r.cur;
     a.Ein = E[c];
-    a.Hin = H[c];
     a.Eout = E[1 - c];
-    a.Hout = H[1 - c];
-    a.psiIn = psi + (c ? PSI_SET : 0) * pstride;
-    a.psiOut = psi + (c ? 0 : PSI_SET) * pstride;
+    a.psiIn = psi + (c ? PSI_SET : 0) * (long long)P;
+    a.psiOut = psi + (c ? 0 : PSI_SET) * (long long)P;
     a.n = n;
     a.out1 = slot(n);
     a.out2 = slot(n + 1);
-    a.cap1 = r.capture ? r.capture + (long long)(n - r.begin + 1) * f3 : nullptr;
-    a.cap2 = r.capture ? r.capture + (long long)(n - r.begin + 2) * f3 : nullptr;
+    a.cap = r.capture ? r.capture + (long long)(n - r.begin + 1) * f3 : nullptr;
     if (cpl == 1) launch2<1, 8, true>(a, s);
     else if (cpl == 4) launch2<4, 4, true>(a, s);
     else if (WY == 4) launch2<2, 4, true>(a, s);
@@ -473,6 +634,83 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
   if (e != hipSuccess || n >= r.end) return e;
   StepRange tail{n, r.end, r.cur, r.capture ? r.capture + (long long)(n - r.begin) * f3 : nullptr};
   return run_stream(g, p, mask, src, win, ws, L, tail, s);
+}
+
+// ---------------------------------------------------------------- batches
+// Simulations of a batch are grouped by cells per lane (zz <= 64: 1, even
+// zz <= 128: 2); each group runs as one persistent launch per step pair.
+bool batch2_supported(const Geom& g) { return g.Z <= 64 || (g.Z <= 128 && g.Z % 2 == 0); }
+
+static constexpr int kBatchWY = 8;
+static constexpr int kBatchLX = 128;
+
+size_t batch2_meta_bytes(const BatchItem* items, int n) {
+  int npairs = 0;
+  for (int i = 0; i < n; ++i) npairs = std::max(npairs, (steps_of(*items[i].p) + 1) / 2);
+  return align256(sizeof(Stream2Args) * 2 * (size_t)n) + align256(sizeof(int) * (size_t)(n + 2) * 2) +
+         align256(sizeof(int) * 2 * (size_t)std::max(npairs, 1));
+}
+
+hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) {
+  std::vector<int> cls[2];
+  int npairs = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!batch2_supported(items[i].g)) return hipErrorInvalidValue;
+    cls[items[i].g.Z <= 64 ? 0 : 1].push_back(i);
+    npairs = std::max(npairs, (steps_of(*items[i].p) + 1) / 2);
+  }
+  // Host image of the argument views and tile prefix sums, copied once.
+  const size_t args_bytes = align256(sizeof(Stream2Args) * 2 * (size_t)n);
+  const size_t ts_bytes = align256(sizeof(int) * (size_t)(n + 2) * 2);
+  std::vector<char> host(args_bytes + ts_bytes, 0);
+  Stream2Args* hargs = (Stream2Args*)host.data();
+  int* hts = (int*)(host.data() + args_bytes);
+  int total[2] = {0, 0}, base[2] = {0, 0};
+  int off = 0;
+  for (int c = 0; c < 2; ++c) {
+    base[c] = off;
+    int t = 0;
+    for (size_t k = 0; k < cls[c].size(); ++k) {
+      const BatchItem& it = items[cls[c][k]];
+      const int LX = std::min(kBatchLX, it.g.X);
+      for (int cur = 0; cur < 2; ++cur)
+        hargs[2 * (off + k) + cur] =
+            make_args2(it.g, *it.p, it.mask, it.src, it.win, it.ws, *it.L, kBatchWY, LX, cur);
+      hts[off + c + k] = t;
+      t += hargs[2 * (off + k)].ntiles;
+    }
+    hts[off + c + cls[c].size()] = t;
+    total[c] = t;
+    off += (int)cls[c].size();
+  }
+  hipError_t e = hipMemcpyAsync(meta, host.data(), host.size(), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  // The host image must outlive the copy.
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  int* counters = (int*)(meta + args_bytes + ts_bytes);
+  e = launch_zero((float*)counters, 2 * (size_t)std::max(npairs, 1), s);
+  if (e != hipSuccess) return e;
+  const Stream2Args* dargs = (const Stream2Args*)meta;
+  const int* dts = (const int*)(meta + args_bytes);
+  const int cus = device_cus();
+  for (int m = 0; m < npairs; ++m) {
+    for (int c = 0; c < 2; ++c) {
+      const int nc = (int)cls[c].size();
+      if (nc == 0) continue;
+      const int grid = std::min(total[c], cus);
+      const Stream2Args* a = dargs + 2 * base[c];
+      const int* ts = dts + base[c] + c;
+      int* ctr = counters + 2 * m + c;
+      if (c == 0)
+        hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY, true>), dim3(grid),
+                           dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, total[c], ctr, 2 * m);
+      else
+        hipLaunchKernelGGL((k_stream2_batch<2, kBatchWY, true>), dim3(grid),
+                           dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, total[c], ctr, 2 * m);
+    }
+  }
+  return hipGetLastError();
 }
 
 }  // namespace pjz

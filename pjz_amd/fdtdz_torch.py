@@ -240,3 +240,53 @@ def fdtdz(epsilon, dt, source_field, source_waveform, source_position,
   if prob.ckpt_every <= 0:
     prob.ckpt_every = default_ckpt_every(steps_of(prob))
   return FdtdzFunction.apply(epsilon, prob, dev, device)
+
+
+def fdtdz_batch(calls):
+  """Runs several fdtdz() calls, given as keyword dicts, as one batch.
+
+  All simulations advance together through the batched kernels
+  (``pjz_fdtd_batch_forward``): persistent two-step launches over a work queue
+  of (simulation, tile) items.  Returns the list of snapshot tensors, in the
+  order of ``calls``.  Forward only; every call needs zz <= 64 or an even
+  zz <= 128.
+  """
+  n = len(calls)
+  if n == 0:
+    return []
+  probs = (_lib.Problem * n)()
+  devs, outs = [], []
+  device = None
+  for i, kw in enumerate(calls):
+    kw = dict(kw)
+    kw.setdefault("offset", (0, 0, 0))
+    arrays = (kw["epsilon"], kw["source_field"], kw["source_waveform"], kw["absorption_mask"],
+              kw["pml_kappa"], kw["pml_sigma"], kw["pml_alpha"])
+    shapes = [tuple(a.shape) if isinstance(a, torch.Tensor) else tuple(np.shape(a))
+              for a in arrays]
+    if len(shapes[2]) != 2 or shapes[2][1] != 2:
+      raise ValueError(f"call {i}: source_waveform must be (tt, 2), got {shapes[2]}")
+    probs[i] = make_problem(shapes[0], kw["dt"], shapes[1], kw["source_position"], shapes[3],
+                            shapes[4], kw["pml_widths"], kw["output_steps"], shapes[2][0],
+                            kw.get("use_reduced_precision", False), kw.get("launch_params"),
+                            kw["offset"])
+    device = device or _device_of(arrays)
+    dev = [_as_dev(a, device) for a in arrays]
+    devs.append(dev)
+    nn = _lib.num_outputs(probs[i])
+    outs.append(torch.empty((nn, 3, probs[i].ex, probs[i].ey, probs[i].ez), dtype=torch.float32,
+                            device=device))
+  lib = _lib.load()
+  nbytes = ctypes.c_size_t(0)
+  _lib.check(lib.pjz_fdtd_batch_workspace_bytes(probs, n, ctypes.byref(nbytes)),
+             "pjz_fdtd_batch_workspace_bytes")
+  ws = _Workspace(nbytes.value, device)
+  col = lambda j: (ctypes.c_void_p * n)(*[d[j].data_ptr() for d in devs])
+  out_ptrs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+  with torch.cuda.device(device):
+    stream = torch.cuda.current_stream(device).cuda_stream
+    rc = lib.pjz_fdtd_batch_forward(probs, n, col(0), col(3), col(4), col(5), col(6), col(1),
+                                    col(2), out_ptrs, ws.ptr, ctypes.c_size_t(ws.nbytes),
+                                    ctypes.c_void_p(stream))
+  _lib.check(rc, "pjz_fdtd_batch_forward")
+  return outs

@@ -89,3 +89,106 @@ def time_steps(kw):
   """Number of FDTD steps fdtdz() runs: the last snapshot step + 1."""
   s = range(*kw["output_steps"])
   return s[-1] + 1 if len(s) else 0
+
+
+def batch64(steps=4000, seed=0, n=64, device=None, sizes=(128, 192, 256, 384, 512),
+            heights=(64, 128)):
+  """Uneven batch: n simulations, xx, yy drawn from ``sizes`` and zz from
+  ``heights`` (seeded), each with its own Bernoulli(0.5) {2.085, 12.25} design
+  region over 25% of the xy area (a centred square) and 8 z layers, a
+  Gaussian-profile Ey current plane at x = 16 driven by a Gaussian pulse
+  (centre frequency 2pi/20), z-CPML 10 cells per side, absorber 16 cells,
+  ``steps`` steps and 4 snapshots.  Arrays are built with torch on ``device``
+  (the GPU by default) so the batch does not pass through host memory.
+  """
+  import torch
+  rng = np.random.default_rng(seed)
+  dev = device or torch.device("cuda", 0)
+  gen = torch.Generator(device=dev).manual_seed(seed)
+  calls = []
+  for i in range(n):
+    X, Y = (int(v) for v in rng.choice(sizes, 2))
+    Z = int(rng.choice(heights))
+    eps = torch.ones((3, X, Y, Z), dtype=torch.float32, device=dev)
+    zc0 = Z // 2 - 4
+    eps[:, :, :, : Z // 3] = 2.085
+    dx, dy = X // 4, Y // 4  # half side of a square holding 25% of the xy area
+    cx, cy = X // 2, Y // 2
+    pat = torch.rand((2 * dx, 2 * dy), generator=gen, device=dev) < 0.5
+    design = torch.where(pat, 2.085, 12.25).to(torch.float32)
+    eps[:, cx - dx:cx + dx, cy - dy:cy + dy, zc0:zc0 + 8] = design[None, :, :, None]
+    yy_ = torch.arange(Y, device=dev, dtype=torch.float32)[:, None] - cy
+    zz_ = torch.arange(Z, device=dev, dtype=torch.float32)[None, :] - (zc0 + 4)
+    sf = torch.zeros((2, 1, Y, Z), dtype=torch.float32, device=dev)
+    sf[0, 0] = torch.exp(-(yy_ ** 2) / (2 * 12.0 ** 2) - (zz_ ** 2) / (2 * 6.0 ** 2))
+    wave = np.zeros((steps, 2), np.float32)
+    wave[:, 0] = _pulse(steps, 2 * math.pi / 20, 200.0, 50.0)
+    calls.append(dict(
+        epsilon=eps, dt=0.5, source_field=sf, source_waveform=torch.as_tensor(wave, device=dev),
+        source_position=16, absorption_mask=torch.as_tensor(B.absorption_mask(X, Y, 16, 1e-3),
+                                                            device=dev),
+        pml_kappa=torch.ones((Z, 2), device=dev),
+        pml_sigma=torch.as_tensor(B.pml_sigma((10, 10), Z, 0.5, 3.0), device=dev),
+        pml_alpha=torch.full((Z, 2), 0.05, device=dev), pml_widths=(10, 10),
+        output_steps=(steps - 40, steps, 10), use_reduced_precision=False,
+        launch_params=None, offset=(0, 0, 0)))
+  return calls
+
+
+GRAD_OMEGAS = (2 * math.pi / 18, 2 * math.pi / 20, 2 * math.pi / 22)
+
+
+def grad_256(steps=6000, seed=0, xx=256, yy=256, zz=128, monitor_x=220):
+  """Gradient workload: the layered stack of ``layered_64`` scaled to
+  256x256x128 (SiO2 below z=48, a Si strip 48<=z<64, 104<=y<152), +-0.01
+  seeded noise, an Ey current plane at x=40 driven by pjz's ramped sine
+  averaged over 2pi/18, 2pi/20, 2pi/22; z-CPML 10 cells, absorber 16 cells;
+  snapshots on pjz's sampling schedule for the three frequencies.
+
+  Returns (kwargs, mode, omegas): ``mode`` is a seeded Gaussian (Ey, Ez)
+  profile (2, yy, zz) on the plane x = ``monitor_x`` used by
+  :func:`overlap_objective`.
+  """
+  rng = np.random.default_rng(seed)
+  X, Y, Z = xx, yy, zz
+  s = zz / 64.0
+  eps = np.ones((3, X, Y, Z), np.float32)
+  eps[:, :, :, : int(24 * s)] = 2.085
+  y0, y1 = int(26 * yy / 64), int(38 * yy / 64)
+  eps[:, :, y0:y1, int(24 * s):int(32 * s)] = 12.25
+  eps += rng.uniform(-0.01, 0.01, eps.shape).astype(np.float32)
+  sf = np.zeros((2, 1, Y, Z), np.float32)
+  yy_, zz_ = np.meshgrid(np.arange(Y) - (y0 + y1) / 2, np.arange(Z) - 28 * s, indexing="ij")
+  sf[0, 0] = np.exp(-(yy_ ** 2) / (2 * (0.5 * (y1 - y0)) ** 2) - (zz_ ** 2) / (2 * (4 * s) ** 2))
+  omegas = np.array(GRAD_OMEGAS)
+  rs = B.ramped_sin(omegas, 4.0, 4.0, 0.5, steps)
+  wave = np.zeros((steps, 2), np.float32)
+  wave[:, 0] = np.imag(rs)
+  interval = B.sampling_interval(omegas.min(), omegas.max(), 3, 0.5)
+  kw = dict(epsilon=eps, dt=0.5, source_field=sf, source_waveform=wave, source_position=40,
+            absorption_mask=B.absorption_mask(X, Y, 16, 1e-3),
+            pml_kappa=np.ones((Z, 2), np.float32), pml_sigma=B.pml_sigma((10, 10), Z, 0.5, 3.0),
+            pml_alpha=np.full((Z, 2), 0.05, np.float32), pml_widths=(10, 10),
+            output_steps=(steps - 2 * interval * 3 - 1, steps, interval),
+            use_reduced_precision=False, launch_params=None, offset=(0, 0, 0))
+  mode = np.zeros((2, Y, Z), np.float32)
+  cy, cz = (y0 + y1) / 2 + rng.uniform(-2, 2), 28 * s + rng.uniform(-2, 2)
+  mode[0] = np.exp(-((np.arange(Y)[:, None] - cy) ** 2) / (2 * 8.0 ** 2)
+                   - ((np.arange(Z)[None, :] - cz) ** 2) / (2 * 5.0 ** 2))
+  mode[1] = 0.1 * rng.standard_normal((Y, Z)) * mode[0]
+  return kw, mode, omegas
+
+
+def overlap_objective(fields, mode, omegas, output_steps, dt=0.5, monitor_x=220):
+  """sum over omega of |<E(omega) on plane monitor_x, mode>|^2 (torch).
+
+  ``fields`` are fdtdz() snapshots (nn, 3, xx, yy, zz); the projection onto
+  frequencies is field.project (pjz's pinv of the output phases).
+  """
+  import torch
+  from .field import project
+  ef = project(fields, omegas, output_steps, dt)  # (ww, 3, xx, yy, zz) complex
+  plane = ef[:, 1:, monitor_x]                    # (ww, 2, yy, zz): Ey, Ez
+  m = torch.as_tensor(mode, device=fields.device).to(plane.dtype)
+  ov = torch.sum(plane * m[None], dim=(1, 2, 3))
+  return torch.sum(torch.abs(ov) ** 2)
