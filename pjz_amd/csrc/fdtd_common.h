@@ -46,6 +46,7 @@ struct OutWin {
 // Byte offsets of the workspace segments (each 256-byte aligned).
 struct Layout {
   size_t E0, H0, E1, H1, cb, psi, zt;
+  size_t rare;        // 256 B of per-call launch data for the two-step kernel
   // Gradient support (only when ckpt_every > 0):
   size_t ckpt;        // nck checkpoints of ckpt_floats each
   size_t seg;         // (S + 1) x 3N: E before every step of one segment
@@ -80,6 +81,7 @@ inline Layout make_layout(const Geom& g, const pjz_fdtd_problem& p) {
   L.psi = off;
   off += align256(sizeof(float) * PSI_N * P);
   L.zt = off; off += align256(sizeof(float) * ZROWS * (size_t)g.Z);
+  L.rare = off; off += 256;
   L.S = 0;
   L.nck = 0;
   L.ckpt_floats = 6 * (size_t)g.N + 4 * P;

@@ -28,6 +28,19 @@
 
 namespace pjz {
 
+// Data touched only by launches that write snapshots or capture E.  It lives
+// in device memory and is read inside a non-inlined store routine, so the
+// stepping loop does not keep it in (scarce) scalar registers.
+struct Stream2Rare {
+  float* out;     // snapshots (nn, 3, wex, wey, wez) or null
+  float* cap;     // E of the whole domain after step m at cap + (m - cap_begin + 1) * 3N
+  int cap_begin;
+  int wx0, wy0, wz0, wex, wey, wez;
+  int ostart, ostop, ostep;
+};
+
+enum Stream2Flags { SNAP1 = 1, SNAP2 = 2, CAPTURE = 4 };
+
 struct Stream2Args {
   const float* Ein;    // old level: E, and H at Ein + hoff
   float* Eout;         // new level: E, and H at Eout + hoff
@@ -38,16 +51,21 @@ struct Stream2Args {
   const float* mask;
   const float* sfield;
   const float* swave;
-  float* out;  // snapshots (nn, 3, wex, wey, wez) or null
-  float* cap;  // capture of E after step n (whole domain), after n+1 at +3N; or null
+  const Stream2Rare* rare;
   int X, Y, Z, lo, hi, NP, N, hoff, P;
   int saxis, spos, squad;
-  int wx0, wy0, wz0, wex, wey, wez;
   float dt;
-  int n, out1, out2;
+  int n, flags;  // first step of the pair; Stream2Flags of this launch
   int LX, nty, ntiles;
-  int ostart, ostop, ostep, nsteps;  // output schedule and length (batched runs)
+  int nsteps;  // steps of the simulation (batched runs)
 };
+
+// Snapshot slot of a step in the output schedule, or -1.
+__device__ inline int out_slot(const Stream2Rare* r, int step) {
+  if (!r->out || step < r->ostart || step >= r->ostop || (step - r->ostart) % r->ostep != 0)
+    return -1;
+  return (step - r->ostart) / r->ostep;
+}
 
 template <int CPL>
 struct ZCoef {
@@ -214,19 +232,31 @@ __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y
     P.sig[k] = active && e_rows ? a.mask[(k * a.X + x) * a.Y + yc] : 0.0f;
 }
 
-template <int CPL>
-__device__ inline void store_snapshot2(const Stream2Args& a, int slot, int x, int y, int zb,
-                                       const float (&en)[3][CPL]) {
-  if (slot < 0 || x < a.wx0 || x >= a.wx0 + a.wex || y < a.wy0 || y >= a.wy0 + a.wey) return;
+// Writes the snapshot and/or capture of E after `step` for one row at plane x.
+template <int CPL, bool vec>
+__device__ __noinline__ void store_rare(const Stream2Args& a, bool snap, bool cap, int step, int x,
+                                        int y, int zb, const float (&en)[3][CPL]) {
+  const Stream2Rare* r = a.rare;
+  if (snap) {
+    const int slot = out_slot(r, step);
+    if (slot >= 0 && x >= r->wx0 && x < r->wx0 + r->wex && y >= r->wy0 && y < r->wy0 + r->wey) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float* op = a.out + ((((long long)slot * 3 + k) * a.wex + (x - a.wx0)) * a.wey +
-                         (y - a.wy0)) * (long long)a.wez;
+      for (int k = 0; k < 3; ++k) {
+        float* op = r->out + ((((long long)slot * 3 + k) * r->wex + (x - r->wx0)) * r->wey +
+                              (y - r->wy0)) * (long long)r->wez;
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      const int z = zb + c;
-      if (z >= a.wz0 && z < a.wz0 + a.wez) op[z - a.wz0] = en[k][c];
+        for (int c = 0; c < CPL; ++c) {
+          const int z = zb + c;
+          if (z >= r->wz0 && z < r->wz0 + r->wez) op[z - r->wz0] = en[k][c];
+        }
+      }
     }
+  }
+  if (cap && r->cap) {
+    float* base = r->cap + (long long)(step - r->cap_begin + 1) * 3 * a.N;
+    const int o = (x * a.Y + y) * a.Z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) store_col<CPL, vec>(base + (k * a.N + o), en[k], zb, a.Z);
   }
 }
 
@@ -310,14 +340,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
     }
     e_row<CPL>(en1, psE1, s.e0, hn1, hdn, hp, cur.cb, cur.sig, psE, zc, a, a.n, x, y, zb, lane,
                act);
-    if (own && act && x >= xa && x < xb) {
-      if (a.out1 >= 0) store_snapshot2<CPL>(a, a.out1, x, y, zb, en1);
-      if (a.cap) {
-        const int o = (x * a.Y + y) * Z;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.cap + (k * N + o), en1[k], zb, Z);
-      }
-    }
+    if ((a.flags & (SNAP1 | CAPTURE)) && own && act && x >= xa && x < xb)
+      store_rare<CPL, vec>(a, a.flags & SNAP1, a.flags & CAPTURE, a.n, x, y, zb, en1);
   } else {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
@@ -385,11 +409,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
           a.psiOut[3 * a.P + pi] = psE2[1][c];
         }
       }
-      if (a.out2 >= 0) store_snapshot2<CPL>(a, a.out2, x - 1, y, zb, en2);
-      if (a.cap) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.cap + (3 * N + k * N + o), en2[k], zb, Z);
-      }
+      if (a.flags & (SNAP2 | CAPTURE))
+        store_rare<CPL, vec>(a, a.flags & SNAP2, a.flags & CAPTURE, a.n + 1, x - 1, y, zb, en2);
     }
   }
 
@@ -479,12 +500,6 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2(Stream2Args a) {
   stream2_tile<CPL, WY, vec>(a, tile, L);
 }
 
-// Snapshot slot of a step in a simulation's output schedule, or -1.
-__device__ inline int out_slot(const Stream2Args& a, int step) {
-  if (!a.out || step < a.ostart || step >= a.ostop || (step - a.ostart) % a.ostep != 0) return -1;
-  return (step - a.ostart) / a.ostep;
-}
-
 // Batched two-step kernel: persistent workgroups take (simulation, tile) work
 // items from a device counter until the launch's items are exhausted; the
 // simulation of item t is the last s with tile_start[s] <= t (tile counts are
@@ -512,8 +527,8 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_batch(
     Stream2Args a = args[2 * lo + ((step >> 1) & 1)];
     if (step >= a.nsteps) continue;
     a.n = step;
-    a.out1 = out_slot(a, step);
-    a.out2 = out_slot(a, step + 1);
+    a.flags = (out_slot(a.rare, step) >= 0 ? SNAP1 : 0) |
+              (out_slot(a.rare, step + 1) >= 0 ? SNAP2 : 0);
     stream2_tile<CPL, WY, vec>(a, t - tile_start[lo], L);
   }
 }
@@ -538,6 +553,28 @@ static int device_cus() {
 }
 
 // Kernel arguments of one simulation with buffer set `cur` as the old level.
+static Stream2Rare rare_of(const pjz_fdtd_problem& p, const OutWin& win, float* capture,
+                           int cap_begin) {
+  Stream2Rare r;
+  r.out = win.out;
+  r.cap = capture;
+  r.cap_begin = cap_begin;
+  r.wx0 = win.x0;
+  r.wy0 = win.y0;
+  r.wz0 = win.z0;
+  r.wex = win.ex;
+  r.wey = win.ey;
+  r.wez = win.ez;
+  r.ostart = p.out_start;
+  r.ostop = p.out_stop;
+  r.ostep = p.out_step;
+  return r;
+}
+
+__global__ void k_put_rare(Stream2Rare* dst, Stream2Rare v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
 static Stream2Args make_args2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                              const Src& src, const OutWin& win, char* ws, const Layout& L,
                              int WY, int LX, int cur) {
@@ -557,14 +594,7 @@ static Stream2Args make_args2(const Geom& g, const pjz_fdtd_problem& p, const fl
   a.saxis = src.axis;
   a.spos = src.pos;
   a.squad = src.quad;
-  a.out = win.out;
-  a.cap = nullptr;
-  a.wx0 = win.x0;
-  a.wy0 = win.y0;
-  a.wz0 = win.z0;
-  a.wex = win.ex;
-  a.wey = win.ey;
-  a.wez = win.ez;
+  a.rare = nullptr;
   a.X = g.X;
   a.Y = g.Y;
   a.Z = g.Z;
@@ -576,13 +606,10 @@ static Stream2Args make_args2(const Geom& g, const pjz_fdtd_problem& p, const fl
   a.hoff = (int)((L.H0 - L.E0) / sizeof(float));
   a.dt = p.dt;
   a.n = 0;
-  a.out1 = a.out2 = -1;
+  a.flags = 0;
   a.LX = LX;
   a.nty = (g.Y + WY - 1) / WY;
   a.ntiles = a.nty * ((g.X + LX - 1) / LX);
-  a.ostart = p.out_start;
-  a.ostop = p.out_stop;
-  a.ostep = p.out_step;
   a.nsteps = steps_of(p);
   return a;
 }
@@ -601,6 +628,10 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
     LX = std::max(8, (g.X + ntx - 1) / ntx);
   }
   Stream2Args a = make_args2(g, p, mask, src, win, ws, L, WY, LX, 0);
+  Stream2Rare* rare = (Stream2Rare*)(ws + L.rare);
+  hipLaunchKernelGGL(k_put_rare, dim3(1), dim3(1), 0, s, rare,
+                     rare_of(p, win, r.capture, r.begin));
+  a.rare = rare;
   float* psi = (float*)(ws + L.psi);
   const int P = g.X * g.Y * g.NP;
   float* E[2] = {(float*)(ws + L.E0), (float*)(ws + L.E1)};
@@ -619,9 +650,8 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
     a.psiIn = psi + (c ? PSI_SET : 0) * (long long)P;
     a.psiOut = psi + (c ? 0 : PSI_SET) * (long long)P;
     a.n = n;
-    a.out1 = slot(n);
-    a.out2 = slot(n + 1);
-    a.cap = r.capture ? r.capture + (long long)(n - r.begin + 1) * f3 : nullptr;
+    a.flags = (slot(n) >= 0 ? SNAP1 : 0) | (slot(n + 1) >= 0 ? SNAP2 : 0) |
+              (r.capture ? CAPTURE : 0);
     if (cpl == 1) launch2<1, 8, true>(a, s);
     else if (cpl == 4) launch2<4, 4, true>(a, s);
     else if (WY == 4) launch2<2, 4, true>(a, s);
@@ -647,7 +677,8 @@ static constexpr int kBatchLX = 128;
 size_t batch2_meta_bytes(const BatchItem* items, int n) {
   int npairs = 0;
   for (int i = 0; i < n; ++i) npairs = std::max(npairs, (steps_of(*items[i].p) + 1) / 2);
-  return align256(sizeof(Stream2Args) * 2 * (size_t)n) + align256(sizeof(int) * (size_t)(n + 2) * 2) +
+  return align256(sizeof(Stream2Args) * 2 * (size_t)n) + align256(sizeof(Stream2Rare) * (size_t)n) +
+         align256(sizeof(int) * (size_t)(n + 2) * 2) +
          align256(sizeof(int) * 2 * (size_t)std::max(npairs, 1));
 }
 
@@ -661,10 +692,13 @@ hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) 
   }
   // Host image of the argument views and tile prefix sums, copied once.
   const size_t args_bytes = align256(sizeof(Stream2Args) * 2 * (size_t)n);
+  const size_t rare_bytes = align256(sizeof(Stream2Rare) * (size_t)n);
   const size_t ts_bytes = align256(sizeof(int) * (size_t)(n + 2) * 2);
-  std::vector<char> host(args_bytes + ts_bytes, 0);
+  std::vector<char> host(args_bytes + rare_bytes + ts_bytes, 0);
   Stream2Args* hargs = (Stream2Args*)host.data();
-  int* hts = (int*)(host.data() + args_bytes);
+  Stream2Rare* hrare = (Stream2Rare*)(host.data() + args_bytes);
+  Stream2Rare* drare = (Stream2Rare*)(meta + args_bytes);
+  int* hts = (int*)(host.data() + args_bytes + rare_bytes);
   int total[2] = {0, 0}, base[2] = {0, 0};
   int off = 0;
   for (int c = 0; c < 2; ++c) {
@@ -673,9 +707,12 @@ hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) 
     for (size_t k = 0; k < cls[c].size(); ++k) {
       const BatchItem& it = items[cls[c][k]];
       const int LX = std::min(kBatchLX, it.g.X);
-      for (int cur = 0; cur < 2; ++cur)
+      hrare[off + k] = rare_of(*it.p, it.win, nullptr, 0);
+      for (int cur = 0; cur < 2; ++cur) {
         hargs[2 * (off + k) + cur] =
             make_args2(it.g, *it.p, it.mask, it.src, it.win, it.ws, *it.L, kBatchWY, LX, cur);
+        hargs[2 * (off + k) + cur].rare = drare + off + k;
+      }
       hts[off + c + k] = t;
       t += hargs[2 * (off + k)].ntiles;
     }
@@ -688,11 +725,11 @@ hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) 
   // The host image must outlive the copy.
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
-  int* counters = (int*)(meta + args_bytes + ts_bytes);
+  int* counters = (int*)(meta + args_bytes + rare_bytes + ts_bytes);
   e = launch_zero((float*)counters, 2 * (size_t)std::max(npairs, 1), s);
   if (e != hipSuccess) return e;
   const Stream2Args* dargs = (const Stream2Args*)meta;
-  const int* dts = (const int*)(meta + args_bytes);
+  const int* dts = (const int*)(meta + args_bytes + rare_bytes);
   const int cus = device_cus();
   for (int m = 0; m < npairs; ++m) {
     for (int c = 0; c < 2; ++c) {

@@ -50,7 +50,12 @@ def test_uneven_batch_matches_oracle_and_single_runs():
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert err <= 1e-4, (case, err)
     single = F.fdtdz(**dict(kw, launch_params=dict(kernel="stream2"))).cpu().numpy()
-    np.testing.assert_array_equal(got, single)
+    if workloads.time_steps(kw) % 2 == 0:
+      np.testing.assert_array_equal(got, single)
+    else:
+      # An odd last step runs as level 1 of a pair in the batch but in the
+      # one-step kernel alone: same formulas, different instruction order.
+      assert np.linalg.norm(got - single) <= 1e-6 * np.linalg.norm(single)
 
 
 def test_batch64_builder_small():
