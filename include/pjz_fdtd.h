@@ -112,7 +112,9 @@ PJZ_FDTD_API int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilo
  * arguments are HOST arrays of n device pointers.  No counterpart in pjz,
  * which runs one fdtdz() per port in a Python list comprehension
  * (src/pjz/_field.py:354-356); this is the batched form of that loop.
- * Every problem needs zz <= 64 or an even zz <= 128.
+ * Every problem needs zz <= 64 or an even zz <= 128.  With ckpt_every > 0
+ * (the same even value in every problem) the call also saves checkpoints for
+ * pjz_fdtd_batch_backward in the same workspace.
  */
 PJZ_FDTD_API int pjz_fdtd_batch_workspace_bytes(const pjz_fdtd_problem* probs, int n,
                                                 size_t* bytes);
@@ -122,6 +124,22 @@ PJZ_FDTD_API int pjz_fdtd_batch_forward(const pjz_fdtd_problem* probs, int n,
                            const float* const* pml_alpha, const float* const* source_field,
                            const float* const* source_waveform, float* const* out,
                            void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Batched gradient: pjz_fdtd_backward for every simulation of a preceding
+ * pjz_fdtd_batch_forward call with ckpt_every > 0 (same problems, arrays and
+ * workspace).  Segments are recomputed by the batched two-step kernel and
+ * every reverse step is one launch pair for the whole batch.  grad_out and
+ * grad_eps are HOST arrays of n device pointers.  No counterpart in pjz (see
+ * pjz_fdtd_backward).
+ */
+PJZ_FDTD_API int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n,
+                            const float* const* epsilon, const float* const* absorption_mask,
+                            const float* const* pml_kappa, const float* const* pml_sigma,
+                            const float* const* pml_alpha, const float* const* source_field,
+                            const float* const* source_waveform, const float* const* grad_out,
+                            float* const* grad_eps, void* workspace, size_t workspace_bytes,
+                            void* stream);
 
 /*
  * Gradient of sum_s <grad_out[s], out[s]> with respect to epsilon, by the

@@ -25,6 +25,7 @@ EXPORTS = (
     "pjz_fdtd_backward",
     "pjz_fdtd_batch_workspace_bytes",
     "pjz_fdtd_batch_forward",
+    "pjz_fdtd_batch_backward",
 )
 
 
@@ -80,6 +81,9 @@ def load():
     lib.pjz_fdtd_batch_workspace_bytes.argtypes = [P, ctypes.c_int, ctypes.POINTER(sz)]
     lib.pjz_fdtd_batch_forward.restype = ctypes.c_int
     lib.pjz_fdtd_batch_forward.argtypes = [P, ctypes.c_int] + [ctypes.POINTER(vp)] * 8 + [vp, sz, vp]
+    lib.pjz_fdtd_batch_backward.restype = ctypes.c_int
+    lib.pjz_fdtd_batch_backward.argtypes = ([P, ctypes.c_int] + [ctypes.POINTER(vp)] * 9 +
+                                            [vp, sz, vp])
     if lib.pjz_fdtd_abi_version() != ABI_VERSION:
       raise RuntimeError("libpjzfdtd.so ABI version mismatch; rebuild it")
     _lib = lib

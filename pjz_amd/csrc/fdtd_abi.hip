@@ -234,20 +234,21 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
 
 namespace {
 
-// Problem with gradient support switched off (batches are forward only).
-pjz_fdtd_problem forward_only(const pjz_fdtd_problem& p) {
-  pjz_fdtd_problem q = p;
-  q.ckpt_every = 0;
-  return q;
-}
+// Workspace plan of a batch: one prepared single-simulation layout per item
+// (with checkpoints and adjoint buffers when ckpt_every > 0), then the batch
+// tables.
+struct BatchPlanHost {
+  std::vector<pjz::Layout> Ls;
+  std::vector<size_t> offs;
+  size_t meta_off = 0, adj_off = 0, total = 0;
+  int S = 0;  // common checkpoint spacing, 0 = forward only
+};
 
-int batch_plan(const pjz_fdtd_problem* probs, int n, std::vector<pjz_fdtd_problem>& qs,
-               std::vector<pjz::Layout>& Ls, std::vector<size_t>& offs, size_t& meta_off,
-               size_t& total) {
+int batch_plan(const pjz_fdtd_problem* probs, int n, BatchPlanHost& bp) {
   if (!probs || n < 1) return fail(PJZ_FDTD_EINVAL, "a batch needs at least one problem");
-  qs.resize(n);
-  Ls.resize(n);
-  offs.resize(n);
+  bp.Ls.resize(n);
+  bp.offs.resize(n);
+  bp.S = probs[0].ckpt_every > 0 ? probs[0].ckpt_every : 0;
   size_t off = 0;
   for (int i = 0; i < n; ++i) {
     int rc = check(&probs[i]);
@@ -256,19 +257,74 @@ int batch_plan(const pjz_fdtd_problem* probs, int n, std::vector<pjz_fdtd_proble
       snprintf(msg, sizeof(msg), "%s", g_err);
       return fail(rc, "batch item %d: %s", i, msg);
     }
-    qs[i] = forward_only(probs[i]);
-    pjz::Geom g = geom_of(qs[i]);
+    const int S = probs[i].ckpt_every > 0 ? probs[i].ckpt_every : 0;
+    if (S != bp.S)
+      return fail(PJZ_FDTD_EINVAL, "batch item %d: ckpt_every %d differs from item 0's %d", i, S,
+                  bp.S);
+    if (S % 2 != 0)
+      return fail(PJZ_FDTD_EINVAL, "batch item %d: ckpt_every must be even, got %d", i, S);
+    pjz::Geom g = geom_of(probs[i]);
     if (!pjz::batch2_supported(g))
       return fail(PJZ_FDTD_EINVAL, "batch item %d: zz must be <= 64 or even and <= 128, got %d", i,
                   g.Z);
-    Ls[i] = pjz::make_layout(g, qs[i]);
-    offs[i] = off;
-    off += pjz::align256(Ls[i].total);
+    bp.Ls[i] = pjz::make_layout(g, probs[i]);
+    bp.offs[i] = off;
+    off += pjz::align256(bp.Ls[i].total);
   }
-  meta_off = off;
+  bp.meta_off = off;
   std::vector<pjz::BatchItem> items(n);
-  for (int i = 0; i < n; ++i) items[i] = {geom_of(qs[i]), &qs[i], nullptr, {}, {}, nullptr, &Ls[i]};
-  total = off + pjz::batch2_meta_bytes(items.data(), n);
+  for (int i = 0; i < n; ++i)
+    items[i] = {geom_of(probs[i]), &probs[i], nullptr, {}, {}, nullptr, &bp.Ls[i], nullptr};
+  bp.adj_off = off + pjz::align256(pjz::batch2_meta_bytes(items.data(), n));
+  bp.total = bp.adj_off + (bp.S > 0 ? pjz::adjoint_batch_meta_bytes(n) : 0);
+  return PJZ_FDTD_OK;
+}
+
+// Shared checks of the batched entry points; on success `items` holds the
+// prepared simulations (coefficients built; state cleared, or adjoint
+// buffers cleared when `adjoint`).
+int batch_setup(const pjz_fdtd_problem* probs, int n, const BatchPlanHost& bp,
+                const float* const* epsilon, const float* const* absorption_mask,
+                const float* const* pml_kappa, const float* const* pml_sigma,
+                const float* const* pml_alpha, const float* const* source_field,
+                const float* const* source_waveform, float* const* out, void* workspace,
+                size_t workspace_bytes, bool adjoint, hipStream_t s,
+                std::vector<pjz::BatchItem>& items) {
+  if (!epsilon || !absorption_mask || !pml_kappa || !pml_sigma || !pml_alpha || !source_field ||
+      !source_waveform || !out || !workspace)
+    return fail(PJZ_FDTD_EINVAL, "a required pointer array is NULL");
+  if (workspace_bytes < bp.total)
+    return fail(PJZ_FDTD_ESPACE, "workspace of %zu bytes, %zu needed", workspace_bytes, bp.total);
+  if (((uintptr_t)workspace & 255) != 0)
+    return fail(PJZ_FDTD_EINVAL, "workspace must be 256-byte aligned");
+  char* base = (char*)workspace;
+  items.resize(n);
+  for (int i = 0; i < n; ++i) {
+    if (!epsilon[i] || !absorption_mask[i] || !pml_kappa[i] || !pml_sigma[i] || !pml_alpha[i] ||
+        !source_field[i] || !source_waveform[i] || !out[i])
+      return fail(PJZ_FDTD_EINVAL, "batch item %d: a required pointer is NULL", i);
+    const pjz_fdtd_problem& q = probs[i];
+    pjz::Geom g = geom_of(q);
+    char* ws = base + bp.offs[i];
+    const pjz::Layout& L = bp.Ls[i];
+    hipError_t e = pjz::launch_prepare(g, q, epsilon[i], absorption_mask[i], pml_kappa[i],
+                                       pml_sigma[i], pml_alpha[i], (float*)(ws + L.cb),
+                                       (float*)(ws + L.zt), s);
+    if (adjoint) {
+      // Clear lE, lH, lu, lpE, A, B (contiguous from L.lE to the end).
+      if (e == hipSuccess && L.total > L.lE)
+        e = pjz::launch_zero((float*)(ws + L.lE), (L.total - L.lE) / 4, s);
+    } else {
+      if (e == hipSuccess) e = pjz::launch_zero((float*)(ws + L.E0), (L.E1 - L.E0) / 4, s);
+      if (e == hipSuccess && g.NP > 0)
+        e = pjz::launch_zero((float*)(ws + L.psi), (L.zt - L.psi) / 4, s);
+    }
+    if (e != hipSuccess) return hip_fail(e, "batch set-up");
+    items[i] = {g, &probs[i], absorption_mask[i],
+                pjz::Src{q.src_axis, q.src_pos, q.src_quad, source_field[i], source_waveform[i]},
+                pjz::OutWin{q.x0, q.y0, q.z0, q.ex, q.ey, q.ez, adjoint ? nullptr : out[i]}, ws,
+                &bp.Ls[i], adjoint && L.nck > 0 ? (float*)(ws + L.seg) : nullptr};
+  }
   return PJZ_FDTD_OK;
 }
 
@@ -277,14 +333,11 @@ int batch_plan(const pjz_fdtd_problem* probs, int n, std::vector<pjz_fdtd_proble
 extern "C" {
 
 int pjz_fdtd_batch_workspace_bytes(const pjz_fdtd_problem* probs, int n, size_t* bytes) {
-  std::vector<pjz_fdtd_problem> qs;
-  std::vector<pjz::Layout> Ls;
-  std::vector<size_t> offs;
-  size_t meta_off = 0, total = 0;
-  int rc = batch_plan(probs, n, qs, Ls, offs, meta_off, total);
+  BatchPlanHost bp;
+  int rc = batch_plan(probs, n, bp);
   if (rc) return rc;
   if (!bytes) return fail(PJZ_FDTD_EINVAL, "bytes is NULL");
-  *bytes = total;
+  *bytes = bp.total;
   return PJZ_FDTD_OK;
 }
 
@@ -294,43 +347,97 @@ int pjz_fdtd_batch_forward(const pjz_fdtd_problem* probs, int n, const float* co
                            const float* const* source_field,
                            const float* const* source_waveform, float* const* out,
                            void* workspace, size_t workspace_bytes, void* stream) {
-  std::vector<pjz_fdtd_problem> qs;
-  std::vector<pjz::Layout> Ls;
-  std::vector<size_t> offs;
-  size_t meta_off = 0, total = 0;
-  int rc = batch_plan(probs, n, qs, Ls, offs, meta_off, total);
+  BatchPlanHost bp;
+  int rc = batch_plan(probs, n, bp);
   if (rc) return rc;
-  if (!epsilon || !absorption_mask || !pml_kappa || !pml_sigma || !pml_alpha || !source_field ||
-      !source_waveform || !out || !workspace)
-    return fail(PJZ_FDTD_EINVAL, "a required pointer array is NULL");
-  if (workspace_bytes < total)
-    return fail(PJZ_FDTD_ESPACE, "workspace of %zu bytes, %zu needed", workspace_bytes, total);
-  if (((uintptr_t)workspace & 255) != 0)
-    return fail(PJZ_FDTD_EINVAL, "workspace must be 256-byte aligned");
   hipStream_t s = (hipStream_t)stream;
+  std::vector<pjz::BatchItem> items;
+  rc = batch_setup(probs, n, bp, epsilon, absorption_mask, pml_kappa, pml_sigma, pml_alpha,
+                   source_field, source_waveform, out, workspace, workspace_bytes, false, s, items);
+  if (rc) return rc;
   char* base = (char*)workspace;
-  std::vector<pjz::BatchItem> items(n);
-  for (int i = 0; i < n; ++i) {
-    if (!epsilon[i] || !absorption_mask[i] || !pml_kappa[i] || !pml_sigma[i] || !pml_alpha[i] ||
-        !source_field[i] || !source_waveform[i] || !out[i])
-      return fail(PJZ_FDTD_EINVAL, "batch item %d: a required pointer is NULL", i);
-    const pjz_fdtd_problem& q = qs[i];
-    pjz::Geom g = geom_of(q);
-    char* ws = base + offs[i];
-    const pjz::Layout& L = Ls[i];
-    hipError_t e = pjz::launch_prepare(g, q, epsilon[i], absorption_mask[i], pml_kappa[i],
-                                       pml_sigma[i], pml_alpha[i], (float*)(ws + L.cb),
-                                       (float*)(ws + L.zt), s);
-    if (e == hipSuccess) e = pjz::launch_zero((float*)(ws + L.E0), (L.E1 - L.E0) / 4, s);
-    if (e == hipSuccess && g.NP > 0)
-      e = pjz::launch_zero((float*)(ws + L.psi), (L.zt - L.psi) / 4, s);
-    if (e != hipSuccess) return hip_fail(e, "batch set-up");
-    items[i] = {g, &qs[i], absorption_mask[i],
-                pjz::Src{q.src_axis, q.src_pos, q.src_quad, source_field[i], source_waveform[i]},
-                pjz::OutWin{q.x0, q.y0, q.z0, q.ex, q.ey, q.ez, out[i]}, ws, &Ls[i]};
+  pjz::Batch2Plan plan;
+  hipError_t e = pjz::batch2_prepare(items.data(), n, base + bp.meta_off, plan, s);
+  if (e == hipSuccess && bp.S == 0) e = pjz::batch2_pairs(plan, 0, plan.npairs, s);
+  // With checkpoints: before the pairs of segment k, save the state of step
+  // k*S (buffer set (k*S/2) & 1) of every simulation that reaches it.
+  for (int k = 0; bp.S > 0 && e == hipSuccess && k * bp.S < 2 * plan.npairs; ++k) {
+    const int b = k * bp.S;
+    for (int i = 0; i < n && e == hipSuccess; ++i)
+      if (k < bp.Ls[i].nck)
+        e = copy_state(items[i].g, bp.Ls[i], items[i].ws, (b / 2) & 1, k, true, s);
+    if (e == hipSuccess) e = pjz::batch2_pairs(plan, b / 2, (b + bp.S) / 2, s);
   }
-  hipError_t e = pjz::run_batch2(items.data(), n, base + meta_off, s);
   if (e != hipSuccess) return hip_fail(e, "batch launch");
+  return PJZ_FDTD_OK;
+}
+
+int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n, const float* const* epsilon,
+                            const float* const* absorption_mask, const float* const* pml_kappa,
+                            const float* const* pml_sigma, const float* const* pml_alpha,
+                            const float* const* source_field,
+                            const float* const* source_waveform, const float* const* grad_out,
+                            float* const* grad_eps, void* workspace, size_t workspace_bytes,
+                            void* stream) {
+  BatchPlanHost bp;
+  int rc = batch_plan(probs, n, bp);
+  if (rc) return rc;
+  if (bp.S <= 0)
+    return fail(PJZ_FDTD_EINVAL,
+                "batch backward needs ckpt_every > 0 (and a batch forward call with it)");
+  if (!grad_out || !grad_eps) return fail(PJZ_FDTD_EINVAL, "a required pointer array is NULL");
+  for (int i = 0; i < n; ++i)
+    if (!grad_out[i] || !grad_eps[i])
+      return fail(PJZ_FDTD_EINVAL, "batch item %d: a required pointer is NULL", i);
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<pjz::BatchItem> items;
+  rc = batch_setup(probs, n, bp, epsilon, absorption_mask, pml_kappa, pml_sigma, pml_alpha,
+                   source_field, source_waveform, grad_eps, workspace, workspace_bytes, true, s,
+                   items);
+  if (rc) return rc;
+  char* base = (char*)workspace;
+  pjz::Batch2Plan plan;
+  pjz::AdjBatchPlan aplan;
+  hipError_t e = pjz::batch2_prepare(items.data(), n, base + bp.meta_off, plan, s);
+  if (e == hipSuccess) e = pjz::adjoint_batch_prepare(items.data(), n, base + bp.adj_off, aplan, s);
+  const int last_seg = (2 * plan.npairs - 1) / bp.S;
+  for (int k = last_seg; k >= 0 && e == hipSuccess; --k) {
+    const int b = k * bp.S;
+    // Recompute segment k of every simulation that reaches it from its
+    // checkpoint, capturing E before each step (slot 0 = E^b).
+    int end = b;
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+      const pjz::Layout& L = bp.Ls[i];
+      if (k >= L.nck) continue;
+      const int c = (b / 2) & 1;
+      end = std::max(end, std::min(b + bp.S, pjz::steps_of(probs[i])));
+      e = copy_state(items[i].g, L, items[i].ws, c, k, false, s);
+      if (e == hipSuccess)
+        e = pjz::launch_copy(items[i].capture, (const float*)(items[i].ws + (c ? L.E1 : L.E0)),
+                             3 * (size_t)items[i].g.N, s);
+    }
+    if (e == hipSuccess) e = pjz::batch2_capture_from(plan, b, s);
+    if (e == hipSuccess) e = pjz::batch2_pairs(plan, b / 2, (end + 1) / 2, s);
+    // Reverse steps end-1 .. b of the whole batch.
+    for (int m = end - 1; m >= b && e == hipSuccess; --m) {
+      for (int i = 0; i < n && e == hipSuccess; ++i) {
+        const pjz_fdtd_problem& q = probs[i];
+        if (m < pjz::steps_of(q) && m >= q.out_start && m < q.out_stop &&
+            (m - q.out_start) % q.out_step == 0) {
+          const long long gstride = 3LL * q.ex * q.ey * q.ez;
+          e = pjz::launch_inject(items[i].g, q, (float*)(items[i].ws + bp.Ls[i].lE),
+                                 grad_out[i] + gstride * ((m - q.out_start) / q.out_step), s);
+        }
+      }
+      if (e == hipSuccess) e = pjz::launch_adjoint_batch_step(aplan, m, b, s);
+    }
+  }
+  for (int i = 0; i < n && e == hipSuccess; ++i)
+    e = bp.Ls[i].nck == 0  // no steps: nothing depends on epsilon
+            ? pjz::launch_zero(grad_eps[i], 3 * (size_t)probs[i].ex * probs[i].ey * probs[i].ez, s)
+            : pjz::launch_grad_eps(items[i].g, probs[i], epsilon[i], absorption_mask[i],
+                                   items[i].ws, bp.Ls[i], grad_eps[i], s);
+  if (e != hipSuccess) return hip_fail(e, "batch backward launch");
   return PJZ_FDTD_OK;
 }
 

@@ -49,7 +49,7 @@ struct Layout {
   size_t rare;        // 256 B of per-call launch data for the two-step kernel
   // Gradient support (only when ckpt_every > 0):
   size_t ckpt;        // nck checkpoints of ckpt_floats each
-  size_t seg;         // (S + 1) x 3N: E before every step of one segment
+  size_t seg;         // (S + 2) x 3N: E before every step of one segment (+1 spare)
   size_t lE, lH;      // adjoint E and H (3N each)
   size_t lu, lpE;     // psi_H and psi_E cotangents (2P each)
   size_t accA, accB;  // gradient accumulators (3N each)
@@ -91,7 +91,9 @@ inline Layout make_layout(const Geom& g, const pjz_fdtd_problem& p) {
     L.S = p.ckpt_every < nsteps ? p.ckpt_every : nsteps;
     L.nck = (nsteps + L.S - 1) / L.S;
     L.ckpt = off; off += align256(sizeof(float) * L.ckpt_floats) * (size_t)L.nck;
-    L.seg = off; off += f3 * (size_t)(L.S + 1);
+    // E^b .. E^(b+S) of a segment, plus one slot for the extra step a batched
+    // pair runs past an odd step count.
+    L.seg = off; off += f3 * (size_t)(L.S + 2);
     L.lE = off; off += f3;
     L.lH = off; off += f3;
     L.lu = off; off += p2;
@@ -142,7 +144,9 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
                        const Src& src, const OutWin& win, char* ws, const Layout& L,
                        const StepRange& r, hipStream_t s);
 
-// One simulation of a batch: its problem, arrays and workspace.
+// One simulation of a batch: its problem, arrays and workspace.  `capture`
+// (or null) receives E of the whole domain after each step, as in StepRange,
+// counted from the step set by batch2_capture_from.
 struct BatchItem {
   Geom g;
   const pjz_fdtd_problem* p;
@@ -151,14 +155,26 @@ struct BatchItem {
   OutWin win;
   char* ws;
   const Layout* L;
+  float* capture;
 };
 
-// Batched two-step stepping (all steps of every simulation).  The items'
-// workspaces must be prepared (coefficients, cleared state); `meta` is device
-// scratch of batch2_meta_bytes() for argument views, tile tables and queue
-// counters.
+// Batched two-step stepping.  The items' workspaces must be prepared
+// (coefficients, cleared state); `meta` is device scratch of
+// batch2_meta_bytes() for argument views, tile tables and queue counters.
+// Pair m advances every simulation from step 2m to 2m+2 and reads the buffer
+// set m & 1 (a simulation skips pairs past its last step).
+struct Batch2Plan {
+  int n, npairs, cus;
+  int count[2], base[2], total[2];  // per cells-per-lane class: sims, first sim, tiles
+  char* meta;
+  size_t args_bytes, rare_bytes, ts_bytes;
+};
 bool batch2_supported(const Geom& g);
 size_t batch2_meta_bytes(const BatchItem* items, int n);
+hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan& plan,
+                          hipStream_t s);
+hipError_t batch2_pairs(const Batch2Plan& plan, int m0, int m1, hipStream_t s);
+hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s);
 hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s);
 
 // Reverse of step n: lE, lH, lu, lpE hold the cotangents after step n on
@@ -166,6 +182,20 @@ hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s);
 hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                                char* ws, const Layout& L, const float* En, const float* En1,
                                hipStream_t s);
+
+// Batched reverse steps: one launch per half step for every simulation of a
+// batch (blocks prefix-summed over simulations).  Each item's E after steps
+// seg_begin-1 .. is in its `capture` buffer (slot n - seg_begin for E^n); a
+// simulation skips steps at or past its last.
+struct AdjBatchPlan {
+  int n, blocks;
+  char* meta;  // device table of per-simulation arguments and first blocks
+};
+size_t adjoint_batch_meta_bytes(int n);
+hipError_t adjoint_batch_prepare(const BatchItem* items, int n, char* meta, AdjBatchPlan& plan,
+                                 hipStream_t s);
+hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int step, int seg_begin,
+                                     hipStream_t s);
 
 // lE[window] += g (one snapshot cotangent, (3, ex, ey, ez)).
 hipError_t launch_inject(const Geom& g, const pjz_fdtd_problem& p, float* lE, const float* g_out,

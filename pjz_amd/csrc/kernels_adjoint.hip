@@ -11,6 +11,8 @@
 // u is the psi_H cotangent before the bH scaling that the next reverse step
 // applies, so no kernel reads a neighbour's psi value another thread writes.
 // Since cb K = E' - ca E, the gradient pieces are gca = A, gcb = (B - ca A)/cb.
+#include <vector>
+
 #include "fdtd_common.h"
 
 namespace pjz {
@@ -41,10 +43,8 @@ struct AdjArgs {
   float dt;
 };
 
-__global__ void k_adj_h(AdjArgs a) {
-  const Geom g = a.g;
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx >= g.N) return;
+__device__ inline void adj_h_cell(const AdjArgs& a, long long idx) {
+  const Geom& g = a.g;
   const long long N = g.N, YZ = (long long)g.Y * g.Z, P = (long long)g.X * g.Y * g.NP;
   const int k = (int)(idx % g.Z);
   const long long xy = idx / g.Z;
@@ -91,10 +91,9 @@ __global__ void k_adj_h(AdjArgs a) {
   }
 }
 
-__global__ void k_adj_e(AdjArgs a) {
-  const Geom g = a.g;
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx >= g.N) return;
+__device__ inline void adj_e_cell(const AdjArgs& a, long long idx, const float* En,
+                                  const float* En1) {
+  const Geom& g = a.g;
   const long long N = g.N, YZ = (long long)g.Y * g.Z, P = (long long)g.X * g.Y * g.NP;
   const int k = (int)(idx % g.Z);
   const long long xy = idx / g.Z;
@@ -137,8 +136,8 @@ __global__ void k_adj_e(AdjArgs a) {
     const float le = a.lE[o];
     const float b = a.cb[o];
     const float s = a.mask[((long long)c * g.X + i) * g.Y + j];
-    a.A[o] += le * a.En[o];
-    a.B[o] += le * a.En1[o];
+    a.A[o] += le * En[o];
+    a.B[o] += le * En1[o];
     lk[c] = b * le;
     a.lE[o] = (1.0f - s * b) * le + gsum[c];
   }
@@ -150,11 +149,19 @@ __global__ void k_adj_e(AdjArgs a) {
   }
 }
 
-hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
-                               char* ws, const Layout& L, const float* En, const float* En1,
-                               hipStream_t s) {
+__global__ void k_adj_h(AdjArgs a) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx < a.g.N) adj_h_cell(a, idx);
+}
+
+__global__ void k_adj_e(AdjArgs a) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx < a.g.N) adj_e_cell(a, idx, a.En, a.En1);
+}
+
+static AdjArgs adj_args(const Geom& g, const pjz_fdtd_problem& p, const float* mask, char* ws,
+                        const Layout& L) {
   AdjArgs a;
-  const size_t P = (size_t)g.X * g.Y * (size_t)(g.NP > 0 ? g.NP : 1);
   a.lE = (float*)(ws + L.lE);
   a.lH = (float*)(ws + L.lH);
   a.lu = (float*)(ws + L.lu);
@@ -164,14 +171,93 @@ hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const f
   a.cb = (const float*)(ws + L.cb);
   a.zt = (const float*)(ws + L.zt);
   a.mask = mask;
-  a.En = En;
-  a.En1 = En1;
+  a.En = nullptr;
+  a.En1 = nullptr;
   a.g = g;
   a.dt = p.dt;
-  (void)P;
+  return a;
+}
+
+hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                               char* ws, const Layout& L, const float* En, const float* En1,
+                               hipStream_t s) {
+  AdjArgs a = adj_args(g, p, mask, ws, L);
+  a.En = En;
+  a.En1 = En1;
   const unsigned nb = nblocks(g.N, 256);
   hipLaunchKernelGGL(k_adj_h, dim3(nb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_adj_e, dim3(nb), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- batches
+// One table entry per simulation: its arguments (En holds the capture base),
+// its step count and its first block in the batch launch.
+struct AdjItem {
+  AdjArgs a;
+  int nsteps;
+  int block0;
+};
+
+// Simulation of block b: the last entry with block0 <= b.
+__device__ inline int adj_item_of(const AdjItem* items, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_adj_h_batch(const AdjItem* __restrict__ items, int n, int step) {
+  const AdjItem& it = items[adj_item_of(items, n, blockIdx.x)];
+  if (step >= it.nsteps) return;
+  const long long idx = (blockIdx.x - it.block0) * (long long)blockDim.x + threadIdx.x;
+  if (idx < it.a.g.N) adj_h_cell(it.a, idx);
+}
+
+__global__ void k_adj_e_batch(const AdjItem* __restrict__ items, int n, int step, int seg_begin) {
+  const AdjItem& it = items[adj_item_of(items, n, blockIdx.x)];
+  if (step >= it.nsteps) return;
+  const long long idx = (blockIdx.x - it.block0) * (long long)blockDim.x + threadIdx.x;
+  if (idx >= it.a.g.N) return;
+  const long long f3 = 3 * it.a.g.N;
+  const float* En = it.a.En + (long long)(step - seg_begin) * f3;
+  adj_e_cell(it.a, idx, En, En + f3);
+}
+
+size_t adjoint_batch_meta_bytes(int n) { return align256(sizeof(AdjItem) * (size_t)n); }
+
+hipError_t adjoint_batch_prepare(const BatchItem* items, int n, char* meta, AdjBatchPlan& plan,
+                                 hipStream_t s) {
+  std::vector<AdjItem> host(n);
+  long long blocks = 0;
+  for (int i = 0; i < n; ++i) {
+    const BatchItem& it = items[i];
+    host[i].a = adj_args(it.g, *it.p, it.mask, it.ws, *it.L);
+    host[i].a.En = it.capture;
+    host[i].nsteps = steps_of(*it.p);
+    host[i].block0 = (int)blocks;
+    blocks += nblocks(it.g.N, 256);
+  }
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  plan.n = n;
+  plan.blocks = (int)blocks;
+  plan.meta = meta;
+  hipError_t e = hipMemcpyAsync(meta, host.data(), sizeof(AdjItem) * (size_t)n,
+                                hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  // The host table must outlive the copy.
+  return hipStreamSynchronize(s);
+}
+
+hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int step, int seg_begin,
+                                     hipStream_t s) {
+  const AdjItem* items = (const AdjItem*)plan.meta;
+  hipLaunchKernelGGL(k_adj_h_batch, dim3(plan.blocks), dim3(256), 0, s, items, plan.n, step);
+  hipLaunchKernelGGL(k_adj_e_batch, dim3(plan.blocks), dim3(256), 0, s, items, plan.n, step,
+                     seg_begin);
   return hipGetLastError();
 }
 

@@ -165,7 +165,9 @@ __device__ inline void e_row(float (&en)[3][CPL], float (&psEn)[2][CPL], const f
     cy[c] = tHx - (hn[2][c] - hp[1][c]);
     cz[c] = (hn[1][c] - hp[0][c]) - (hn[0][c] - hdn[0][c]);
   }
-  if (valid) add_source2<CPL>(a, step, x, y, zb, cx, cy, cz);
+  // A batched pair may run one step past the simulation's last (odd step
+  // counts); the waveform has no row for it and the result is never read.
+  if (valid && step < a.nsteps) add_source2<CPL>(a, step, x, y, zb, cx, cy, cz);
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     en[0][c] = (1.0f - sig[0] * cb[0][c]) * e[0][c] + cb[0][c] * cx[c];
@@ -528,7 +530,7 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_batch(
     if (step >= a.nsteps) continue;
     a.n = step;
     a.flags = (out_slot(a.rare, step) >= 0 ? SNAP1 : 0) |
-              (out_slot(a.rare, step + 1) >= 0 ? SNAP2 : 0);
+              (out_slot(a.rare, step + 1) >= 0 ? SNAP2 : 0) | (a.rare->cap ? CAPTURE : 0);
     stream2_tile<CPL, WY, vec>(a, t - tile_start[lo], L);
   }
 }
@@ -682,32 +684,38 @@ size_t batch2_meta_bytes(const BatchItem* items, int n) {
          align256(sizeof(int) * 2 * (size_t)std::max(npairs, 1));
 }
 
-hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) {
+hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan& plan,
+                          hipStream_t s) {
   std::vector<int> cls[2];
-  int npairs = 0;
+  plan.n = n;
+  plan.npairs = 0;
+  plan.meta = meta;
+  plan.cus = device_cus();
   for (int i = 0; i < n; ++i) {
     if (!batch2_supported(items[i].g)) return hipErrorInvalidValue;
     cls[items[i].g.Z <= 64 ? 0 : 1].push_back(i);
-    npairs = std::max(npairs, (steps_of(*items[i].p) + 1) / 2);
+    plan.npairs = std::max(plan.npairs, (steps_of(*items[i].p) + 1) / 2);
   }
-  // Host image of the argument views and tile prefix sums, copied once.
-  const size_t args_bytes = align256(sizeof(Stream2Args) * 2 * (size_t)n);
-  const size_t rare_bytes = align256(sizeof(Stream2Rare) * (size_t)n);
-  const size_t ts_bytes = align256(sizeof(int) * (size_t)(n + 2) * 2);
-  std::vector<char> host(args_bytes + rare_bytes + ts_bytes, 0);
+  // Host image of the argument views, launch data and tile prefix sums,
+  // copied once.  Layout: args (2 per sim, class order) | rare (class order)
+  // | tile starts (n_c + 1 per class) | queue counters (2 per pair).
+  plan.args_bytes = align256(sizeof(Stream2Args) * 2 * (size_t)n);
+  plan.rare_bytes = align256(sizeof(Stream2Rare) * (size_t)n);
+  plan.ts_bytes = align256(sizeof(int) * (size_t)(n + 2) * 2);
+  std::vector<char> host(plan.args_bytes + plan.rare_bytes + plan.ts_bytes, 0);
   Stream2Args* hargs = (Stream2Args*)host.data();
-  Stream2Rare* hrare = (Stream2Rare*)(host.data() + args_bytes);
-  Stream2Rare* drare = (Stream2Rare*)(meta + args_bytes);
-  int* hts = (int*)(host.data() + args_bytes + rare_bytes);
-  int total[2] = {0, 0}, base[2] = {0, 0};
+  Stream2Rare* hrare = (Stream2Rare*)(host.data() + plan.args_bytes);
+  Stream2Rare* drare = (Stream2Rare*)(meta + plan.args_bytes);
+  int* hts = (int*)(host.data() + plan.args_bytes + plan.rare_bytes);
   int off = 0;
   for (int c = 0; c < 2; ++c) {
-    base[c] = off;
+    plan.base[c] = off;
+    plan.count[c] = (int)cls[c].size();
     int t = 0;
     for (size_t k = 0; k < cls[c].size(); ++k) {
       const BatchItem& it = items[cls[c][k]];
       const int LX = std::min(kBatchLX, it.g.X);
-      hrare[off + k] = rare_of(*it.p, it.win, nullptr, 0);
+      hrare[off + k] = rare_of(*it.p, it.win, it.capture, 0);
       for (int cur = 0; cur < 2; ++cur) {
         hargs[2 * (off + k) + cur] =
             make_args2(it.g, *it.p, it.mask, it.src, it.win, it.ws, *it.L, kBatchWY, LX, cur);
@@ -717,37 +725,62 @@ hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) 
       t += hargs[2 * (off + k)].ntiles;
     }
     hts[off + c + cls[c].size()] = t;
-    total[c] = t;
+    plan.total[c] = t;
     off += (int)cls[c].size();
   }
   hipError_t e = hipMemcpyAsync(meta, host.data(), host.size(), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return e;
   // The host image must outlive the copy.
-  e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return e;
-  int* counters = (int*)(meta + args_bytes + rare_bytes + ts_bytes);
-  e = launch_zero((float*)counters, 2 * (size_t)std::max(npairs, 1), s);
+  return hipStreamSynchronize(s);
+}
+
+hipError_t batch2_pairs(const Batch2Plan& plan, int m0, int m1, hipStream_t s) {
+  m1 = std::min(m1, plan.npairs);
+  if (m0 >= m1) return hipSuccess;
+  char* meta = plan.meta;
+  int* counters = (int*)(meta + plan.args_bytes + plan.rare_bytes + plan.ts_bytes);
+  // Queue counters of these pairs start at zero (a fill kernel, so the
+  // sequence can be captured in a graph).
+  hipError_t e = launch_zero((float*)(counters + 2 * m0), 2 * (size_t)(m1 - m0), s);
   if (e != hipSuccess) return e;
   const Stream2Args* dargs = (const Stream2Args*)meta;
-  const int* dts = (const int*)(meta + args_bytes + rare_bytes);
-  const int cus = device_cus();
-  for (int m = 0; m < npairs; ++m) {
+  const int* dts = (const int*)(meta + plan.args_bytes + plan.rare_bytes);
+  for (int m = m0; m < m1; ++m) {
     for (int c = 0; c < 2; ++c) {
-      const int nc = (int)cls[c].size();
+      const int nc = plan.count[c];
       if (nc == 0) continue;
-      const int grid = std::min(total[c], cus);
-      const Stream2Args* a = dargs + 2 * base[c];
-      const int* ts = dts + base[c] + c;
+      const int grid = std::min(plan.total[c], plan.cus);
+      const Stream2Args* a = dargs + 2 * plan.base[c];
+      const int* ts = dts + plan.base[c] + c;
       int* ctr = counters + 2 * m + c;
       if (c == 0)
         hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY, true>), dim3(grid),
-                           dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, total[c], ctr, 2 * m);
+                           dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, plan.total[c], ctr, 2 * m);
       else
         hipLaunchKernelGGL((k_stream2_batch<2, kBatchWY, true>), dim3(grid),
-                           dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, total[c], ctr, 2 * m);
+                           dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, plan.total[c], ctr, 2 * m);
     }
   }
   return hipGetLastError();
+}
+
+__global__ void k_capture_from(Stream2Rare* r, int n, int begin) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i].cap_begin = begin;
+}
+
+hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s) {
+  Stream2Rare* r = (Stream2Rare*)(plan.meta + plan.args_bytes);
+  hipLaunchKernelGGL(k_capture_from, dim3((plan.n + 255) / 256), dim3(256), 0, s, r, plan.n,
+                     begin);
+  return hipGetLastError();
+}
+
+hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) {
+  Batch2Plan plan;
+  hipError_t e = batch2_prepare(items, n, meta, plan, s);
+  if (e != hipSuccess) return e;
+  return batch2_pairs(plan, 0, plan.npairs, s);
 }
 
 }  // namespace pjz

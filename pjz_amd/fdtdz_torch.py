@@ -242,20 +242,11 @@ def fdtdz(epsilon, dt, source_field, source_waveform, source_position,
   return FdtdzFunction.apply(epsilon, prob, dev, device)
 
 
-def fdtdz_batch(calls):
-  """Runs several fdtdz() calls, given as keyword dicts, as one batch.
-
-  All simulations advance together through the batched kernels
-  (``pjz_fdtd_batch_forward``): persistent two-step launches over a work queue
-  of (simulation, tile) items.  Returns the list of snapshot tensors, in the
-  order of ``calls``.  Forward only; every call needs zz <= 64 or an even
-  zz <= 128.
-  """
+def _batch_problems(calls):
+  """(probs ctypes array, per-call device arrays, device) of a batch."""
   n = len(calls)
-  if n == 0:
-    return []
   probs = (_lib.Problem * n)()
-  devs, outs = [], []
+  devs = []
   device = None
   for i, kw in enumerate(calls):
     kw = dict(kw)
@@ -270,23 +261,108 @@ def fdtdz_batch(calls):
                             shapes[4], kw["pml_widths"], kw["output_steps"], shapes[2][0],
                             kw.get("use_reduced_precision", False), kw.get("launch_params"),
                             kw["offset"])
+    _lib.num_outputs(probs[i])
     device = device or _device_of(arrays)
-    dev = [_as_dev(a, device) for a in arrays]
-    devs.append(dev)
-    nn = _lib.num_outputs(probs[i])
-    outs.append(torch.empty((nn, 3, probs[i].ex, probs[i].ey, probs[i].ez), dtype=torch.float32,
-                            device=device))
+    devs.append([_as_dev(a, device) for a in arrays])
+  return probs, devs, device
+
+
+def _batch_workspace(probs, device):
   lib = _lib.load()
   nbytes = ctypes.c_size_t(0)
-  _lib.check(lib.pjz_fdtd_batch_workspace_bytes(probs, n, ctypes.byref(nbytes)),
+  _lib.check(lib.pjz_fdtd_batch_workspace_bytes(probs, len(probs), ctypes.byref(nbytes)),
              "pjz_fdtd_batch_workspace_bytes")
-  ws = _Workspace(nbytes.value, device)
-  col = lambda j: (ctypes.c_void_p * n)(*[d[j].data_ptr() for d in devs])
-  out_ptrs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+  return _Workspace(nbytes.value, device)
+
+
+def _ptrs(tensors):
+  return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+
+
+def _batch_forward(probs, devs, device):
+  """Runs pjz_fdtd_batch_forward; returns (outs, workspace)."""
+  n = len(probs)
+  outs = [torch.empty((_lib.num_outputs(p), 3, p.ex, p.ey, p.ez), dtype=torch.float32,
+                      device=device) for p in probs]
+  ws = _batch_workspace(probs, device)
+  col = lambda j: _ptrs([d[j] for d in devs])
   with torch.cuda.device(device):
     stream = torch.cuda.current_stream(device).cuda_stream
-    rc = lib.pjz_fdtd_batch_forward(probs, n, col(0), col(3), col(4), col(5), col(6), col(1),
-                                    col(2), out_ptrs, ws.ptr, ctypes.c_size_t(ws.nbytes),
-                                    ctypes.c_void_p(stream))
+    rc = _lib.load().pjz_fdtd_batch_forward(
+        probs, n, col(0), col(3), col(4), col(5), col(6), col(1), col(2), _ptrs(outs), ws.ptr,
+        ctypes.c_size_t(ws.nbytes), ctypes.c_void_p(stream))
   _lib.check(rc, "pjz_fdtd_batch_forward")
-  return outs
+  return outs, ws
+
+
+def _batch_backward(probs, devs, device, ws, grad_outs):
+  n = len(probs)
+  grads = [torch.empty((3, p.ex, p.ey, p.ez), dtype=torch.float32, device=device) for p in probs]
+  gouts = [torch.zeros((_lib.num_outputs(p), 3, p.ex, p.ey, p.ez), dtype=torch.float32,
+                       device=device) if g is None else
+           g.detach().to(device=device, dtype=torch.float32).contiguous()
+           for p, g in zip(probs, grad_outs)]
+  col = lambda j: _ptrs([d[j] for d in devs])
+  with torch.cuda.device(device):
+    stream = torch.cuda.current_stream(device).cuda_stream
+    rc = _lib.load().pjz_fdtd_batch_backward(
+        probs, n, col(0), col(3), col(4), col(5), col(6), col(1), col(2), _ptrs(gouts),
+        _ptrs(grads), ws.ptr, ctypes.c_size_t(ws.nbytes), ctypes.c_void_p(stream))
+  _lib.check(rc, "pjz_fdtd_batch_backward")
+  return grads
+
+
+class FdtdzBatchFunction(torch.autograd.Function):
+  """A batch of fdtdz() calls with gradients for every ``epsilon``."""
+
+  @staticmethod
+  def forward(ctx, probs, devs, device, *epsilons):
+    outs, ws = _batch_forward(probs, devs, device)
+    ctx.probs, ctx.devs, ctx.device, ctx.ws = probs, devs, device, ws
+    ctx.eps_devices = [e.device for e in epsilons]
+    return tuple(outs)
+
+  @staticmethod
+  def backward(ctx, *grad_outs):
+    grads = _batch_backward(ctx.probs, ctx.devs, ctx.device, ctx.ws, grad_outs)
+    ctx.ws = None
+    return (None, None, None) + tuple(
+        g.to(d) if need else None
+        for g, d, need in zip(grads, ctx.eps_devices, ctx.needs_input_grad[3:]))
+
+
+def batch_ckpt_every(probs):
+  """Common even checkpoint spacing of a differentiated batch: the largest
+  ``ckpt_every`` the calls ask for, else about sqrt(2 * steps) of the longest."""
+  asked = max(p.ckpt_every for p in probs)
+  s = asked if asked > 0 else default_ckpt_every(max(steps_of(p) for p in probs))
+  return s + (s % 2)
+
+
+def fdtdz_batch(calls):
+  """Runs several fdtdz() calls, given as keyword dicts, as one batch.
+
+  All simulations advance together through the batched kernels
+  (``pjz_fdtd_batch_forward``): persistent two-step launches over a work queue
+  of (simulation, tile) items.  Returns the list of snapshot tensors, in the
+  order of ``calls``.  When any ``epsilon`` requires grad, the batch is one
+  ``torch.autograd.Function`` whose backward runs the batched adjoint
+  (``pjz_fdtd_batch_backward``) for every simulation at once.  Every call needs
+  zz <= 64 or an even zz <= 128.
+  """
+  if len(calls) == 0:
+    return []
+  probs, devs, device = _batch_problems(calls)
+  epsilons = [kw["epsilon"] for kw in calls]
+  wants_grad = torch.is_grad_enabled() and any(
+      isinstance(e, torch.Tensor) and e.requires_grad for e in epsilons)
+  if not wants_grad:
+    for p in probs:
+      p.ckpt_every = 0
+    return _batch_forward(probs, devs, device)[0]
+  S = batch_ckpt_every(probs)
+  for p in probs:
+    p.ckpt_every = S
+  eps_in = [e if isinstance(e, torch.Tensor) else torch.as_tensor(np.asarray(e))
+            for e in epsilons]
+  return list(FdtdzBatchFunction.apply(probs, devs, device, *eps_in))

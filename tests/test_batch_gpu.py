@@ -70,3 +70,57 @@ def test_batch_rejects_unsupported_height():
   kw = sim(9, 20, 20, 129, 30, 0)
   with pytest.raises(ValueError, match="zz"):
     F.fdtdz_batch([kw])
+
+
+def grad_case(seed, X, Y, Z, tt, axis):
+  kw = sim(seed, X, Y, Z, tt, axis)
+  rng = np.random.default_rng(100 + seed)
+  nn = len(range(*kw["output_steps"]))
+  g = rng.standard_normal((nn,) + kw["epsilon"].shape).astype(np.float32)
+  return kw, g
+
+
+# Uneven sizes, both cells-per-lane classes, odd and even step counts, every
+# source axis; step counts 61/70/53 put segment ends inside the last pair.
+GRAD_CASES = [(11, 24, 18, 40, 70, 0), (12, 30, 21, 64, 61, 1), (13, 20, 23, 96, 70, 2),
+              (14, 19, 30, 24, 53, 2)]
+
+
+@pytest.mark.parametrize("ckpt", [8, 22])
+def test_batch_gradient_matches_single_and_oracle(ckpt):
+  cases = [grad_case(*c) for c in GRAD_CASES]
+  eps = [torch.tensor(kw["epsilon"], device="cuda", requires_grad=True) for kw, _ in cases]
+  calls = [dict(kw, epsilon=e, launch_params=dict(ckpt_every=ckpt))
+           for (kw, _), e in zip(cases, eps)]
+  outs = F.fdtdz_batch(calls)
+  loss = sum((o * torch.as_tensor(g, device="cuda")).sum() for o, (_, g) in zip(outs, cases))
+  loss.backward()
+  torch.cuda.synchronize()
+  for case, e, (kw, g), out in zip(GRAD_CASES, eps, cases, outs):
+    print(f"case {case} ckpt {ckpt}", flush=True)
+    # The same simulation alone, differentiated by the single-call adjoint.
+    e1 = torch.tensor(kw["epsilon"], device="cuda", requires_grad=True)
+    o1 = F.fdtdz(**dict(kw, epsilon=e1, launch_params=dict(kernel="stream2", ckpt_every=ckpt)))
+    (o1 * torch.as_tensor(g, device="cuda")).sum().backward()
+    got, single = e.grad.cpu().numpy(), e1.grad.cpu().numpy()
+    assert np.linalg.norm(got - single) <= 1e-5 * np.linalg.norm(single), case
+    assert np.linalg.norm(out.detach().cpu().numpy() - o1.detach().cpu().numpy()) <= \
+        1e-6 * np.linalg.norm(o1.detach().cpu().numpy())
+  # Two cases against the FP64 discrete adjoint oracle.
+  from oracle import adjoint
+  for i in (0, 3):
+    kw, g = cases[i]
+    ref = adjoint.grad_epsilon(kw["epsilon"], g_out=g,
+                               **{k: v for k, v in kw.items() if k != "epsilon"})
+    got = eps[i].grad.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-3 * np.linalg.norm(ref), GRAD_CASES[i]
+
+
+def test_batch_gradient_only_where_asked():
+  (kw0, g0), (kw1, g1) = grad_case(21, 20, 18, 40, 50, 0), grad_case(22, 22, 20, 64, 50, 1)
+  e0 = torch.tensor(kw0["epsilon"], device="cuda", requires_grad=True)
+  outs = F.fdtdz_batch([dict(kw0, epsilon=e0), kw1])
+  (outs[0] * torch.as_tensor(g0, device="cuda")).sum().backward()
+  ref = F.fdtdz(**dict(kw0, launch_params=dict(kernel="stream2"))).cpu().numpy()
+  assert np.linalg.norm(outs[0].detach().cpu().numpy() - ref) <= 1e-6 * np.linalg.norm(ref)
+  assert e0.grad is not None and torch.isfinite(e0.grad).all()
