@@ -13,8 +13,10 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                        [--kernel auto|reference|stream|stream2] [--no-cpu-baseline]
 
 Other workloads (one JSON line each): --workload grad_256 (forward +
-adjoint gradient, 256x256x128, 6000 steps) and --workload batch64 (64
-uneven simulations batched against 64 sequential calls).
+adjoint gradient, 256x256x128, 6000 steps), --workload batch64 (64 uneven
+simulations batched against 64 sequential calls) and --workload
+design_batch (64 simulations, forward + gradient, batched against
+sequential calls).
 """
 
 import argparse
@@ -165,6 +167,76 @@ def bench_batch(args):
   print(json.dumps(line), flush=True)
 
 
+def bench_design(args):
+  """Batched inverse-design iteration: 16 variants x 2 ports x 2 frequencies
+  (64 simulations), forward + adjoint gradient of the summed overlap
+  objective, as one batch against 64 sequential single-simulation calls."""
+  import torch
+  from pjz_amd import fdtdz, fdtdz_batch, workloads
+  steps = args.time_steps or 2000
+  calls, variant_of, monitors = workloads.design_batch(steps=steps)
+  nv = max(variant_of) + 1
+  base_eps = [None] * nv
+  for c, v in zip(calls, variant_of):
+    base_eps[v] = c["epsilon"]
+  cells = sum(int(c["epsilon"].shape[1] * c["epsilon"].shape[2] * c["epsilon"].shape[3])
+              for c in calls)
+
+  def objective(i, out):
+    w, mon_x, mode = monitors[i]
+    return workloads.overlap_objective(out, mode, [w], calls[i]["output_steps"],
+                                       monitor_x=mon_x)
+
+  def leaves():
+    return [e.detach().clone().requires_grad_(True) for e in base_eps]
+
+  def batched():
+    lv = leaves()
+    outs = fdtdz_batch([dict(c, epsilon=lv[v]) for c, v in zip(calls, variant_of)])
+    sum(objective(i, o) for i, o in enumerate(outs)).backward()
+    return [e.grad for e in lv]
+
+  def sequential(kernel):
+    lv = leaves()
+    for i, (c, v) in enumerate(zip(calls, variant_of)):
+      out = fdtdz(**dict(c, epsilon=lv[v], launch_params=dict(kernel=kernel)))
+      objective(i, out).backward()
+    return [e.grad for e in lv]
+
+  def timed(fn, *a):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+      r = fn(*a)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / args.steps, r
+
+  for _ in range(args.warmup):
+    batched()
+    sequential("auto")
+  tb, gb = timed(batched)
+  ts, gs = timed(sequential, "auto")
+  tn, _ = timed(sequential, "reference")
+  diff = max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+             for a, b in zip(gb, gs))
+  shape = calls[0]["epsilon"].shape
+  line = {"metric": METRIC, "value": round(cells * steps / tb / 1e9, 3),
+          "unit": "Gcell-updates/s (forward steps per second of forward + gradient)",
+          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+          "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+          "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+          "config": {"workload": f"design_batch: 16 variants x 2 ports x 2 frequencies = "
+                                 f"{len(calls)} simulations of {shape[1]}x{shape[2]}x{shape[3]} "
+                                 f"cells, {steps} steps, forward + adjoint gradient of the "
+                                 f"overlap objective (reduced from 384x384x128, 8000 steps)",
+                     "cells": cells, "time_steps": steps, "batched_s": round(tb, 4),
+                     "sequential_best_s": round(ts, 4), "sequential_naive_s": round(tn, 4),
+                     "speedup_vs_sequential_best": round(ts / tb, 3),
+                     "speedup_vs_sequential_naive_one_step_kernel": round(tn / tb, 3),
+                     "grad_rel_l2_batched_vs_sequential": diff}}
+  print(json.dumps(line), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +257,8 @@ def main():
     return bench_grad(args)
   if args.workload == "batch64":
     return bench_batch(args)
+  if args.workload == "design_batch":
+    return bench_design(args)
 
   import numpy as np
   import torch

@@ -192,3 +192,65 @@ def overlap_objective(fields, mode, omegas, output_steps, dt=0.5, monitor_x=220)
   m = torch.as_tensor(mode, device=fields.device).to(plane.dtype)
   ov = torch.sum(plane * m[None], dim=(1, 2, 3))
   return torch.sum(torch.abs(ov) ** 2)
+
+
+def design_batch(steps=2000, seed=0, variants=16, xx=128, yy=128, zz=64, device=None):
+  """Batched inverse-design iteration: ``variants`` design variants x 2 ports
+  x 2 frequencies = 4 * variants simulations of one size.
+
+  Each variant's design region (the central 5/16 of x and y, 8 z layers in
+  the core) holds eps = 2.085 + (12.25 - 2.085) * sigmoid(latent) with a
+  seeded uniform(-3, 3) latent, over a SiO2 (z < zz/3) / air stack with a Si
+  strip through the design.  Port 0 injects an Ey Gaussian-profile current
+  plane at x = 16, port 1 at x = xx - 18; frequency k drives pjz's ramped sine
+  at 2pi/18 or 2pi/22 and samples pjz's 3-snapshot schedule for it.  z-CPML
+  10 cells, absorber 12 cells.
+
+  Returns (calls, variant_of, monitors): the fdtdz() keyword dicts (epsilon
+  tensors built with torch on ``device``, the GPU by default), the variant of
+  each call, and per call (omega, monitor x, mode (2, yy, zz)) for
+  :func:`overlap_objective`.
+  """
+  import torch
+  rng = np.random.default_rng(seed)
+  dev = device or torch.device("cuda", 0)
+  X, Y, Z = xx, yy, zz
+  zc0 = Z // 2 - 4
+  dx, dy = (5 * X) // 32, (5 * Y) // 32
+  cx, cy = X // 2, Y // 2
+  base = np.ones((3, X, Y, Z), np.float32)
+  base[:, :, :, : Z // 3] = 2.085
+  base[:, :, cy - 6:cy + 6, zc0:zc0 + 8] = 12.25
+  yy_, zz_ = np.meshgrid(np.arange(Y) - cy, np.arange(Z) - (zc0 + 4), indexing="ij")
+  prof = np.exp(-(yy_ ** 2) / (2 * 6.0 ** 2) - (zz_ ** 2) / (2 * 4.0 ** 2)).astype(np.float32)
+  sf = np.zeros((2, 1, Y, Z), np.float32)
+  sf[0, 0] = prof
+  mode = np.zeros((2, Y, Z), np.float32)
+  mode[0] = prof
+  omegas = (2 * math.pi / 18, 2 * math.pi / 22)
+  mask = B.absorption_mask(X, Y, 12, 2e-3)
+  sigma = B.pml_sigma((10, 10), Z, 0.5, 3.0)
+  calls, variant_of, monitors = [], [], []
+  for v in range(variants):
+    latent = rng.uniform(-3, 3, (2 * dx, 2 * dy)).astype(np.float32)
+    eps = base.copy()
+    eps[:, cx - dx:cx + dx, cy - dy:cy + dy, zc0:zc0 + 8] = (
+        2.085 + (12.25 - 2.085) / (1 + np.exp(-latent)))[None, :, :, None]
+    eps_t = torch.as_tensor(eps, device=dev)
+    for port, (src_x, mon_x) in enumerate(((16, X - 24), (X - 18, 22))):
+      for w in omegas:
+        interval = B.sampling_interval(w, w, 1, 0.5)
+        wave = np.zeros((steps, 2), np.float32)
+        wave[:, 0] = np.imag(B.ramped_sin(np.array([w]), 4.0, 4.0, 0.5, steps))
+        calls.append(dict(
+            epsilon=eps_t, dt=0.5, source_field=torch.as_tensor(sf, device=dev),
+            source_waveform=torch.as_tensor(wave, device=dev), source_position=src_x,
+            absorption_mask=torch.as_tensor(mask, device=dev),
+            pml_kappa=torch.ones((Z, 2), device=dev),
+            pml_sigma=torch.as_tensor(sigma, device=dev),
+            pml_alpha=torch.full((Z, 2), 0.05, device=dev), pml_widths=(10, 10),
+            output_steps=(steps - 2 * interval - 1, steps, interval),
+            use_reduced_precision=False, launch_params=None, offset=(0, 0, 0)))
+        variant_of.append(v)
+        monitors.append((w, mon_x, mode))
+  return calls, variant_of, monitors
