@@ -311,7 +311,7 @@ int batch_setup(const pjz_fdtd_problem* probs, int n, const BatchPlanHost& bp,
                                        pml_sigma[i], pml_alpha[i], (float*)(ws + L.cb),
                                        (float*)(ws + L.zt), s);
     if (adjoint) {
-      // Clear lE, lH, lu, lpE, A, B (contiguous from L.lE to the end).
+      // Clear lE, lH, lu, lpE, G (contiguous from L.lE to the end).
       if (e == hipSuccess && L.total > L.lE)
         e = pjz::launch_zero((float*)(ws + L.lE), (L.total - L.lE) / 4, s);
     } else {
@@ -404,7 +404,7 @@ int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n, const float* c
   for (int k = last_seg; k >= 0 && e == hipSuccess; --k) {
     const int b = k * bp.S;
     // Recompute segment k of every simulation that reaches it from its
-    // checkpoint, capturing E before each step (slot 0 = E^b).
+    // checkpoint, capturing D^n = E^n - E^(n+1) (slot n - b).
     int end = b;
     for (int i = 0; i < n && e == hipSuccess; ++i) {
       const pjz::Layout& L = bp.Ls[i];
@@ -412,9 +412,6 @@ int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n, const float* c
       const int c = (b / 2) & 1;
       end = std::max(end, std::min(b + bp.S, pjz::steps_of(probs[i])));
       e = copy_state(items[i].g, L, items[i].ws, c, k, false, s);
-      if (e == hipSuccess)
-        e = pjz::launch_copy(items[i].capture, (const float*)(items[i].ws + (c ? L.E1 : L.E0)),
-                             3 * (size_t)items[i].g.N, s);
     }
     if (e == hipSuccess) e = pjz::batch2_capture_from(plan, b, s);
     if (e == hipSuccess) e = pjz::batch2_pairs(plan, b / 2, (end + 1) / 2, s);
@@ -472,7 +469,7 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
   }
   e = pjz::launch_prepare(g, *p, epsilon, absorption_mask, pml_kappa, pml_sigma, pml_alpha,
                           (float*)(ws + L.cb), (float*)(ws + L.zt), s);
-  // Clear lE, lH, lu, lpE, A, B (contiguous from L.lE to the end).
+  // Clear lE, lH, lu, lpE, G (contiguous from L.lE to the end).
   if (e == hipSuccess) e = pjz::launch_zero((float*)(ws + L.lE), (L.total - L.lE) / 4, s);
   if (e != hipSuccess) return hip_fail(e, "backward set-up");
   pjz::Src src{p->src_axis, p->src_pos, p->src_quad, source_field, source_waveform};
@@ -483,10 +480,9 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
   (void)P;
   for (int k = L.nck - 1; k >= 0 && e == hipSuccess; --k) {
     const int b = k * L.S, end = b + L.S < nsteps ? b + L.S : nsteps;
-    // Recompute the segment from its checkpoint, capturing E before each step.
+    // Recompute the segment from its checkpoint, capturing D^n = E^n - E^(n+1).
     int cur = 0;
     e = restore_state(g, L, ws, k, s);
-    if (e == hipSuccess) e = pjz::launch_copy(seg, (const float*)(ws + L.E0), f3, s);
     if (e == hipSuccess)
       e = run_steps(g, *p, absorption_mask, src, nowin, ws, L, b, end, &cur, seg, s);
     for (int n = end - 1; n >= b && e == hipSuccess; --n) {
@@ -494,8 +490,7 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
         e = pjz::launch_inject(g, *p, lE, grad_out + gstride * ((n - p->out_start) / p->out_step),
                                s);
       if (e == hipSuccess)
-        e = pjz::launch_adjoint_step(g, *p, absorption_mask, ws, L, seg + (n - b) * f3,
-                                     seg + (n - b + 1) * f3, s);
+        e = pjz::launch_adjoint_step(g, *p, absorption_mask, ws, L, seg + (n - b) * f3, s);
     }
   }
   if (e == hipSuccess)

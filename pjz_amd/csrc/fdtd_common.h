@@ -49,10 +49,10 @@ struct Layout {
   size_t rare;        // 256 B of per-call launch data for the two-step kernel
   // Gradient support (only when ckpt_every > 0):
   size_t ckpt;        // nck checkpoints of ckpt_floats each
-  size_t seg;         // (S + 2) x 3N: E before every step of one segment (+1 spare)
+  size_t seg;         // (S + 1) x 3N: E^n - E^(n+1) of every step of a segment (+1 spare)
   size_t lE, lH;      // adjoint E and H (3N each)
   size_t lu, lpE;     // psi_H and psi_E cotangents (2P each)
-  size_t accA, accB;  // gradient accumulators (3N each)
+  size_t accG;        // gradient accumulator sum_n lE^(n+1) (E^n - E^(n+1)) (3N)
   size_t total;
   int S, nck;
   size_t ckpt_floats;  // E (3N) + H (3N) + psi_H pair (2P) + psi_E pair (2P)
@@ -85,21 +85,20 @@ inline Layout make_layout(const Geom& g, const pjz_fdtd_problem& p) {
   L.S = 0;
   L.nck = 0;
   L.ckpt_floats = 6 * (size_t)g.N + 4 * P;
-  L.ckpt = L.seg = L.lE = L.lH = L.lu = L.lpE = L.accA = L.accB = off;
+  L.ckpt = L.seg = L.lE = L.lH = L.lu = L.lpE = L.accG = off;
   const int nsteps = steps_of(p);
   if (p.ckpt_every > 0 && nsteps > 0) {
     L.S = p.ckpt_every < nsteps ? p.ckpt_every : nsteps;
     L.nck = (nsteps + L.S - 1) / L.S;
     L.ckpt = off; off += align256(sizeof(float) * L.ckpt_floats) * (size_t)L.nck;
-    // E^b .. E^(b+S) of a segment, plus one slot for the extra step a batched
-    // pair runs past an odd step count.
-    L.seg = off; off += f3 * (size_t)(L.S + 2);
+    // D^b .. D^(b+S-1) of a segment, plus one slot for the extra step a
+    // batched pair runs past an odd step count.
+    L.seg = off; off += f3 * (size_t)(L.S + 1);
     L.lE = off; off += f3;
     L.lH = off; off += f3;
     L.lu = off; off += p2;
     L.lpE = off; off += p2;
-    L.accA = off; off += f3;
-    L.accB = off; off += f3;
+    L.accG = off; off += f3;
   }
   L.total = off;
   return L;
@@ -109,8 +108,9 @@ inline size_t ckpt_stride(const Layout& L) { return align256(sizeof(float) * L.c
 
 // A run of steps [begin, end).  *cur names the ping-pong buffer set that holds
 // the state before the run and is updated to the set that holds it after (the
-// reference stepper works in place on set 0).  If capture is set, the E of
-// the whole domain after step n is also written to capture + (n-begin+1)*3N.
+// reference stepper works in place on set 0).  If capture is set, the change
+// of E over step n, D^n = E^n - E^(n+1), of the whole domain is also written
+// to capture + (n-begin)*3N (what the gradient needs of the forward fields).
 struct StepRange {
   int begin, end;
   int* cur;
@@ -178,15 +178,14 @@ hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s)
 hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s);
 
 // Reverse of step n: lE, lH, lu, lpE hold the cotangents after step n on
-// entry and before it on exit; A, B accumulate lE' E^n and lE' E^(n+1).
+// entry and before it on exit; G accumulates lE' D^n (D^n = E^n - E^(n+1)).
 hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
-                               char* ws, const Layout& L, const float* En, const float* En1,
-                               hipStream_t s);
+                               char* ws, const Layout& L, const float* D, hipStream_t s);
 
 // Batched reverse steps: one launch per half step for every simulation of a
-// batch (blocks prefix-summed over simulations).  Each item's E after steps
-// seg_begin-1 .. is in its `capture` buffer (slot n - seg_begin for E^n); a
-// simulation skips steps at or past its last.
+// batch (blocks prefix-summed over simulations).  Each item's D^n is in its
+// `capture` buffer at slot n - seg_begin; a simulation skips steps at or past
+// its last.
 struct AdjBatchPlan {
   int n, blocks;
   char* meta;  // device table of per-simulation arguments and first blocks
@@ -202,7 +201,7 @@ hipError_t launch_inject(const Geom& g, const pjz_fdtd_problem& p, float* lE, co
                          hipStream_t s);
 
 // grad_eps (3, ex, ey, ez) = sum over the cells each window cell fills of
-// A dca/deps + (B - ca A)/cb dcb/deps.
+// (cb / dt) G, which is dL/dca dca/deps + dL/dcb dcb/deps when ca = 1 - s cb.
 hipError_t launch_grad_eps(const Geom& g, const pjz_fdtd_problem& p, const float* eps,
                            const float* mask, const char* ws, const Layout& L, float* grad_eps,
                            hipStream_t s);

@@ -7,10 +7,15 @@
 // Reverse step n, with lK = cb lE':
 //   k_adj_h (H cells): lH <- lH + Cb^T lK - Dzf(cE (lpsiE' -+ lK)),  u <- bH u +- dt lH
 //   k_adj_e (E cells): lE <- ca lE' + dt Cf^T lH - Dzb(cH u),  lpsiE <- bE (lpsiE' -+ lK),
-//                      A += lE' E^n,  B += lE' E^(n+1)
+//                      G += lE' (E^n - E^(n+1))
 // u is the psi_H cotangent before the bH scaling that the next reverse step
 // applies, so no kernel reads a neighbour's psi value another thread writes.
-// Since cb K = E' - ca E, the gradient pieces are gca = A, gcb = (B - ca A)/cb.
+// Gradient: dL/dca = sum lE' E^n = A and dL/dcb = sum lE' K = (B - ca A)/cb
+// with B = sum lE' E^(n+1) (since cb K = E' - ca E).  With ca = 1 - s cb and
+// cb = dt / (eps + s dt / 2): dca/deps = s cb^2/dt, dcb/deps = -cb^2/dt, so
+// dL/deps = (cb/dt) (A - B) = (cb/dt) G.  Accumulating G from the per-step
+// change of E halves the accumulator traffic and avoids the cancellation of
+// A - B.
 #include <vector>
 
 #include "fdtd_common.h"
@@ -32,13 +37,11 @@ struct AdjArgs {
   float* lH;
   float* lu;   // (2, X, Y, NP): psi_H cotangent pair (x, y)
   float* lpE;  // (2, X, Y, NP): psi_E cotangent pair (x, y)
-  float* A;
-  float* B;
+  float* G;
   const float* cb;
   const float* mask;
   const float* zt;
-  const float* En;
-  const float* En1;
+  const float* D;  // E^n - E^(n+1) of the step being reversed
   Geom g;
   float dt;
 };
@@ -91,8 +94,7 @@ __device__ inline void adj_h_cell(const AdjArgs& a, long long idx) {
   }
 }
 
-__device__ inline void adj_e_cell(const AdjArgs& a, long long idx, const float* En,
-                                  const float* En1) {
+__device__ inline void adj_e_cell(const AdjArgs& a, long long idx, const float* D) {
   const Geom& g = a.g;
   const long long N = g.N, YZ = (long long)g.Y * g.Z, P = (long long)g.X * g.Y * g.NP;
   const int k = (int)(idx % g.Z);
@@ -136,8 +138,7 @@ __device__ inline void adj_e_cell(const AdjArgs& a, long long idx, const float* 
     const float le = a.lE[o];
     const float b = a.cb[o];
     const float s = a.mask[((long long)c * g.X + i) * g.Y + j];
-    a.A[o] += le * En[o];
-    a.B[o] += le * En1[o];
+    a.G[o] += le * D[o];
     lk[c] = b * le;
     a.lE[o] = (1.0f - s * b) * le + gsum[c];
   }
@@ -156,7 +157,7 @@ __global__ void k_adj_h(AdjArgs a) {
 
 __global__ void k_adj_e(AdjArgs a) {
   const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx < a.g.N) adj_e_cell(a, idx, a.En, a.En1);
+  if (idx < a.g.N) adj_e_cell(a, idx, a.D);
 }
 
 static AdjArgs adj_args(const Geom& g, const pjz_fdtd_problem& p, const float* mask, char* ws,
@@ -166,24 +167,20 @@ static AdjArgs adj_args(const Geom& g, const pjz_fdtd_problem& p, const float* m
   a.lH = (float*)(ws + L.lH);
   a.lu = (float*)(ws + L.lu);
   a.lpE = (float*)(ws + L.lpE);
-  a.A = (float*)(ws + L.accA);
-  a.B = (float*)(ws + L.accB);
+  a.G = (float*)(ws + L.accG);
   a.cb = (const float*)(ws + L.cb);
   a.zt = (const float*)(ws + L.zt);
   a.mask = mask;
-  a.En = nullptr;
-  a.En1 = nullptr;
+  a.D = nullptr;
   a.g = g;
   a.dt = p.dt;
   return a;
 }
 
 hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
-                               char* ws, const Layout& L, const float* En, const float* En1,
-                               hipStream_t s) {
+                               char* ws, const Layout& L, const float* D, hipStream_t s) {
   AdjArgs a = adj_args(g, p, mask, ws, L);
-  a.En = En;
-  a.En1 = En1;
+  a.D = D;
   const unsigned nb = nblocks(g.N, 256);
   hipLaunchKernelGGL(k_adj_h, dim3(nb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_adj_e, dim3(nb), dim3(256), 0, s, a);
@@ -191,7 +188,7 @@ hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const f
 }
 
 // ---------------------------------------------------------------- batches
-// One table entry per simulation: its arguments (En holds the capture base),
+// One table entry per simulation: its arguments (D holds the capture base),
 // its step count and its first block in the batch launch.
 struct AdjItem {
   AdjArgs a;
@@ -222,9 +219,7 @@ __global__ void k_adj_e_batch(const AdjItem* __restrict__ items, int n, int step
   if (step >= it.nsteps) return;
   const long long idx = (blockIdx.x - it.block0) * (long long)blockDim.x + threadIdx.x;
   if (idx >= it.a.g.N) return;
-  const long long f3 = 3 * it.a.g.N;
-  const float* En = it.a.En + (long long)(step - seg_begin) * f3;
-  adj_e_cell(it.a, idx, En, En + f3);
+  adj_e_cell(it.a, idx, it.a.D + (long long)(step - seg_begin) * 3 * it.a.g.N);
 }
 
 size_t adjoint_batch_meta_bytes(int n) { return align256(sizeof(AdjItem) * (size_t)n); }
@@ -236,7 +231,7 @@ hipError_t adjoint_batch_prepare(const BatchItem* items, int n, char* meta, AdjB
   for (int i = 0; i < n; ++i) {
     const BatchItem& it = items[i];
     host[i].a = adj_args(it.g, *it.p, it.mask, it.ws, *it.L);
-    host[i].a.En = it.capture;
+    host[i].a.D = it.capture;
     host[i].nsteps = steps_of(*it.p);
     host[i].block0 = (int)blocks;
     blocks += nblocks(it.g.N, 256);
@@ -284,9 +279,8 @@ hipError_t launch_inject(const Geom& g, const pjz_fdtd_problem& p, float* lE, co
 }
 
 __global__ void k_grad_eps(const float* __restrict__ eps, const float* __restrict__ mask,
-                           const float* __restrict__ A, const float* __restrict__ B,
-                           float* __restrict__ grad, Geom g, int x0, int y0, int z0, int ex,
-                           int ey, int ez, double dt) {
+                           const float* __restrict__ G, float* __restrict__ grad, Geom g, int x0,
+                           int y0, int z0, int ex, int ey, int ez, double dt) {
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= 3 * g.N) return;
   const int k = (int)(t % g.Z);
@@ -301,14 +295,8 @@ __global__ void k_grad_eps(const float* __restrict__ eps, const float* __restric
   const long long w = (((long long)c * ex + ii) * ey + jj) * ez + kk;
   const double e = eps[w];
   const double s = mask[((long long)c * g.X + i) * g.Y + j];
-  const double q = s * dt / (2.0 * e);
-  const double cb = (dt / e) / (1.0 + q);
-  const double ca = 1.0 - s * (double)(float)cb;  // the coefficient the stepper used
-  const double a = A[t];
-  const double gcb = ((double)B[t] - ca * a) / cb;
-  const double val = a * 2.0 * q / (e * (1.0 + q) * (1.0 + q)) -
-                     gcb * dt / (e * e * (1.0 + q) * (1.0 + q));
-  atomicAdd(&grad[w], (float)val);
+  const double cb = dt / (e + 0.5 * s * dt);
+  atomicAdd(&grad[w], (float)(cb / dt * (double)G[t]));
 }
 
 hipError_t launch_grad_eps(const Geom& g, const pjz_fdtd_problem& p, const float* eps,
@@ -317,8 +305,8 @@ hipError_t launch_grad_eps(const Geom& g, const pjz_fdtd_problem& p, const float
   hipError_t e = launch_zero(grad_eps, 3 * (size_t)p.ex * p.ey * p.ez, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_grad_eps, dim3(nblocks(3 * g.N, 256)), dim3(256), 0, s, eps, mask,
-                     (const float*)(ws + L.accA), (const float*)(ws + L.accB), grad_eps, g, p.x0,
-                     p.y0, p.z0, p.ex, p.ey, p.ez, (double)p.dt);
+                     (const float*)(ws + L.accG), grad_eps, g, p.x0, p.y0, p.z0, p.ex, p.ey,
+                     p.ez, (double)p.dt);
   return hipGetLastError();
 }
 

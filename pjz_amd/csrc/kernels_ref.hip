@@ -188,9 +188,10 @@ __global__ void k_ref_e(float* __restrict__ E, const float* __restrict__ H,
     float b = cb[c * N + idx];
     float s = mask[((long long)c * g.X + i) * g.Y + j];
     float a = 1.0f - s * b;
-    float e = a * E[c * N + idx] + b * c3[c];
+    const float eold = E[c * N + idx];
+    float e = a * eold + b * c3[c];
     E[c * N + idx] = e;
-    if (Ecap) Ecap[c * N + idx] = e;
+    if (Ecap) Ecap[c * N + idx] = eold - e;
     if (in_win) {
       long long o = ((((long long)out_idx * 3 + c) * win.ex + (i - win.x0)) * win.ey +
                      (j - win.y0)) * win.ez + (k - win.z0);
@@ -212,7 +213,7 @@ hipError_t run_reference(const Geom& g, const pjz_fdtd_problem& p, const float* 
     int out_idx = -1;
     if (win.out && n >= p.out_start && n < p.out_stop && (n - p.out_start) % p.out_step == 0)
       out_idx = (n - p.out_start) / p.out_step;
-    float* cap = r.capture ? r.capture + (long long)(n - r.begin + 1) * 3 * g.N : nullptr;
+    float* cap = r.capture ? r.capture + (long long)(n - r.begin) * 3 * g.N : nullptr;
     hipLaunchKernelGGL(k_ref_h, dim3(nb), dim3(256), 0, s, E, H, psi, zt, g, p.dt);
     hipLaunchKernelGGL(k_ref_e, dim3(nb), dim3(256), 0, s, E, H, psi, zt, cb, mask, g, src, n,
                        win, out_idx, cap);

@@ -36,7 +36,7 @@ struct StreamArgs {
   float dt;
   int n;        // time step
   int out_idx;  // snapshot slot or -1
-  float* Ecap;  // if set, the new E of the whole domain is also written here
+  float* Ecap;  // if set, E^n - E^(n+1) of the whole domain is also written here
   int LX;       // x planes per workgroup
   int nty;      // number of y tiles
   int ntiles;   // total tiles
@@ -255,7 +255,12 @@ k_stream(StreamArgs a) {
       for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.Eout + k * N + o, en[k], zb, Z);
       if (a.Ecap) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.Ecap + k * N + o, en[k], zb, Z);
+        for (int k = 0; k < 3; ++k) {
+          float d[CPL];
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) d[c] = e0[k][c] - en[k][c];
+          store_col<CPL, vec>(a.Ecap + k * N + o, d, zb, Z);
+        }
       }
       if (a.out_idx >= 0 && x >= a.win.x0 && x < a.win.x0 + a.win.ex && y >= a.win.y0 &&
           y < a.win.y0 + a.win.ey) {
@@ -295,7 +300,7 @@ static hipError_t launch_stream(StreamArgs& a, const StepRange& r, const pjz_fdt
     a.out_idx = -1;
     if (a.win.out && n >= p.out_start && n < p.out_stop && (n - p.out_start) % p.out_step == 0)
       a.out_idx = (n - p.out_start) / p.out_step;
-    a.Ecap = r.capture ? r.capture + (long long)(n - r.begin + 1) * f3 : nullptr;
+    a.Ecap = r.capture ? r.capture + (long long)(n - r.begin) * f3 : nullptr;
     // The state is read from buffer set *cur and written to the other one.
     const bool even = *r.cur == 0;
     *r.cur = even ? 1 : 0;

@@ -33,7 +33,7 @@ namespace pjz {
 // stepping loop does not keep it in (scarce) scalar registers.
 struct Stream2Rare {
   float* out;     // snapshots (nn, 3, wex, wey, wez) or null
-  float* cap;     // E of the whole domain after step m at cap + (m - cap_begin + 1) * 3N
+  float* cap;     // E^m - E^(m+1) of the whole domain at cap + (m - cap_begin) * 3N
   int cap_begin;
   int wx0, wy0, wz0, wex, wey, wez;
   int ostart, ostop, ostep;
@@ -234,10 +234,12 @@ __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y
     P.sig[k] = active && e_rows ? a.mask[(k * a.X + x) * a.Y + yc] : 0.0f;
 }
 
-// Writes the snapshot and/or capture of E after `step` for one row at plane x.
+// Writes the snapshot of E after `step` (en) and/or the capture of its change
+// over the step (eo - en) for one row at plane x.
 template <int CPL, bool vec>
 __device__ __noinline__ void store_rare(const Stream2Args& a, bool snap, bool cap, int step, int x,
-                                        int y, int zb, const float (&en)[3][CPL]) {
+                                        int y, int zb, const float (&en)[3][CPL],
+                                        const float (&eo)[3][CPL]) {
   const Stream2Rare* r = a.rare;
   if (snap) {
     const int slot = out_slot(r, step);
@@ -255,10 +257,15 @@ __device__ __noinline__ void store_rare(const Stream2Args& a, bool snap, bool ca
     }
   }
   if (cap && r->cap) {
-    float* base = r->cap + (long long)(step - r->cap_begin + 1) * 3 * a.N;
+    float* base = r->cap + (long long)(step - r->cap_begin) * 3 * a.N;
     const int o = (x * a.Y + y) * a.Z;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) store_col<CPL, vec>(base + (k * a.N + o), en[k], zb, a.Z);
+    for (int k = 0; k < 3; ++k) {
+      float d[CPL];
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) d[c] = eo[k][c] - en[k][c];
+      store_col<CPL, vec>(base + (k * a.N + o), d, zb, a.Z);
+    }
   }
 }
 
@@ -343,7 +350,7 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
     e_row<CPL>(en1, psE1, s.e0, hn1, hdn, hp, cur.cb, cur.sig, psE, zc, a, a.n, x, y, zb, lane,
                act);
     if ((a.flags & (SNAP1 | CAPTURE)) && own && act && x >= xa && x < xb)
-      store_rare<CPL, vec>(a, a.flags & SNAP1, a.flags & CAPTURE, a.n, x, y, zb, en1);
+      store_rare<CPL, vec>(a, a.flags & SNAP1, a.flags & CAPTURE, a.n, x, y, zb, en1, s.e0);
   } else {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
@@ -412,7 +419,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
         }
       }
       if (a.flags & (SNAP2 | CAPTURE))
-        store_rare<CPL, vec>(a, a.flags & SNAP2, a.flags & CAPTURE, a.n + 1, x - 1, y, zb, en2);
+        store_rare<CPL, vec>(a, a.flags & SNAP2, a.flags & CAPTURE, a.n + 1, x - 1, y, zb, en2,
+                             s.e1m);
     }
   }
 

@@ -49,6 +49,69 @@ __device__ inline void store_col(float* __restrict__ p, const float (&v)[CPL], i
   }
 }
 
+// Raw buffer access: a 128-bit resource (base, size in bytes) in scalar
+// registers plus 32-bit byte offsets, one vector (VGPR) and one uniform
+// (SGPR).  An offset at or past the resource's size reads 0 and drops the
+// store, so kOff replaces per-lane bounds branches.  Resources here are at
+// most 2^31 bytes, which kOff is never below.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr unsigned kOff = 0x80000000u;
+// Resource word 3 of a raw (untyped, unswizzled) buffer on gfx9-family GPUs.
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ inline rsrc_t make_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           kRsrcWord3);
+}
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline float bload1(rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+__device__ inline void bstore1(float v, rsrc_t r, unsigned voff, unsigned soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
+// CPL consecutive floats at byte offset voff + soff.
+template <int CPL>
+__device__ inline void bload(float (&v)[CPL], rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (CPL == 4) {
+    const u32x4 t = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    v[0] = __builtin_bit_cast(float, t.x);
+    v[1] = __builtin_bit_cast(float, t.y);
+    v[2] = __builtin_bit_cast(float, t.z);
+    v[3] = __builtin_bit_cast(float, t.w);
+  } else if constexpr (CPL == 2) {
+    const u32x2 t = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    v[0] = __builtin_bit_cast(float, t.x);
+    v[1] = __builtin_bit_cast(float, t.y);
+  } else {
+    v[0] = bload1(r, voff, soff);
+  }
+}
+
+template <int CPL>
+__device__ inline void bstore(const float (&v)[CPL], rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (CPL == 4) {
+    u32x4 t;
+    t.x = __builtin_bit_cast(unsigned, v[0]);
+    t.y = __builtin_bit_cast(unsigned, v[1]);
+    t.z = __builtin_bit_cast(unsigned, v[2]);
+    t.w = __builtin_bit_cast(unsigned, v[3]);
+    __builtin_amdgcn_raw_buffer_store_b128(t, r, voff, soff, 0);
+  } else if constexpr (CPL == 2) {
+    u32x2 t;
+    t.x = __builtin_bit_cast(unsigned, v[0]);
+    t.y = __builtin_bit_cast(unsigned, v[1]);
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, voff, soff, 0);
+  } else {
+    bstore1(v[0], r, voff, soff);
+  }
+}
+
 // Cell z+1 for each of the lane's cells (0 at and beyond Z).
 template <int CPL>
 __device__ inline void z_next(float (&o)[CPL], const float (&v)[CPL], int zb, int Z) {
