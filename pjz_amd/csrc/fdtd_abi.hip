@@ -311,9 +311,9 @@ int batch_setup(const pjz_fdtd_problem* probs, int n, const BatchPlanHost& bp,
                                        pml_sigma[i], pml_alpha[i], (float*)(ws + L.cb),
                                        (float*)(ws + L.zt), s);
     if (adjoint) {
-      // Clear lE, lH, lu, lpE, G (contiguous from L.lE to the end).
-      if (e == hipSuccess && L.total > L.lE)
-        e = pjz::launch_zero((float*)(ws + L.lE), (L.total - L.lE) / 4, s);
+      // Clear both adjoint sets and G (contiguous from L.lE_[0] to the end).
+      if (e == hipSuccess && L.total > L.lE_[0])
+        e = pjz::launch_zero((float*)(ws + L.lE_[0]), (L.total - L.lE_[0]) / 4, s);
     } else {
       if (e == hipSuccess) e = pjz::launch_zero((float*)(ws + L.E0), (L.E1 - L.E0) / 4, s);
       if (e == hipSuccess && g.NP > 0)
@@ -422,7 +422,7 @@ int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n, const float* c
         if (m < pjz::steps_of(q) && m >= q.out_start && m < q.out_stop &&
             (m - q.out_start) % q.out_step == 0) {
           const long long gstride = 3LL * q.ex * q.ey * q.ez;
-          e = pjz::launch_inject(items[i].g, q, (float*)(items[i].ws + bp.Ls[i].lE),
+          e = pjz::launch_inject(items[i].g, q, (float*)(items[i].ws + bp.Ls[i].lE_[0]),
                                  grad_out[i] + gstride * ((m - q.out_start) / q.out_step), s);
         }
       }
@@ -469,14 +469,18 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
   }
   e = pjz::launch_prepare(g, *p, epsilon, absorption_mask, pml_kappa, pml_sigma, pml_alpha,
                           (float*)(ws + L.cb), (float*)(ws + L.zt), s);
-  // Clear lE, lH, lu, lpE, G (contiguous from L.lE to the end).
-  if (e == hipSuccess) e = pjz::launch_zero((float*)(ws + L.lE), (L.total - L.lE) / 4, s);
+  // Clear both adjoint sets and G (contiguous from L.lE_[0] to the end).
+  if (e == hipSuccess)
+    e = pjz::launch_zero((float*)(ws + L.lE_[0]), (L.total - L.lE_[0]) / 4, s);
   if (e != hipSuccess) return hip_fail(e, "backward set-up");
+  // The reference kernel choice also selects the cell-by-cell reverse step.
+  const bool stream_adj =
+      p->kernel != PJZ_FDTD_KERNEL_REFERENCE && pjz::adjoint_stream_supported(g);
   pjz::Src src{p->src_axis, p->src_pos, p->src_quad, source_field, source_waveform};
   pjz::OutWin nowin{p->x0, p->y0, p->z0, p->ex, p->ey, p->ez, nullptr};
   float* seg = (float*)(ws + L.seg);
-  float* lE = (float*)(ws + L.lE);
   const long long gstride = 3LL * p->ex * p->ey * p->ez;
+  int acur = 0;  // adjoint set holding the current cotangents
   (void)P;
   for (int k = L.nck - 1; k >= 0 && e == hipSuccess; --k) {
     const int b = k * L.S, end = b + L.S < nsteps ? b + L.S : nsteps;
@@ -487,10 +491,16 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
       e = run_steps(g, *p, absorption_mask, src, nowin, ws, L, b, end, &cur, seg, s);
     for (int n = end - 1; n >= b && e == hipSuccess; --n) {
       if (n >= p->out_start && n < p->out_stop && (n - p->out_start) % p->out_step == 0)
-        e = pjz::launch_inject(g, *p, lE, grad_out + gstride * ((n - p->out_start) / p->out_step),
-                               s);
-      if (e == hipSuccess)
-        e = pjz::launch_adjoint_step(g, *p, absorption_mask, ws, L, seg + (n - b) * f3, s);
+        e = pjz::launch_inject(g, *p, (float*)(ws + L.lE_[acur]),
+                               grad_out + gstride * ((n - p->out_start) / p->out_step), s);
+      if (e != hipSuccess) break;
+      if (stream_adj) {
+        e = pjz::launch_adjoint_stream(g, *p, absorption_mask, ws, L, seg + (n - b) * f3, acur,
+                                       s);
+        acur = 1 - acur;
+      } else {
+        e = pjz::launch_adjoint_step(g, *p, absorption_mask, ws, L, seg + (n - b) * f3, acur, s);
+      }
     }
   }
   if (e == hipSuccess)

@@ -50,8 +50,10 @@ struct Layout {
   // Gradient support (only when ckpt_every > 0):
   size_t ckpt;        // nck checkpoints of ckpt_floats each
   size_t seg;         // (S + 1) x 3N: E^n - E^(n+1) of every step of a segment (+1 spare)
-  size_t lE, lH;      // adjoint E and H (3N each)
-  size_t lu, lpE;     // psi_H and psi_E cotangents (2P each)
+  // Adjoint state in two ping-pong sets (the streaming reverse step reads one
+  // and writes the other; the cell-by-cell one works in place):
+  size_t lE_[2], lH_[2];   // adjoint E and H (3N each)
+  size_t lu_[2], lpE_[2];  // psi_H and psi_E cotangents (2P each)
   size_t accG;        // gradient accumulator sum_n lE^(n+1) (E^n - E^(n+1)) (3N)
   size_t total;
   int S, nck;
@@ -85,7 +87,8 @@ inline Layout make_layout(const Geom& g, const pjz_fdtd_problem& p) {
   L.S = 0;
   L.nck = 0;
   L.ckpt_floats = 6 * (size_t)g.N + 4 * P;
-  L.ckpt = L.seg = L.lE = L.lH = L.lu = L.lpE = L.accG = off;
+  L.ckpt = L.seg = L.accG = off;
+  for (int i = 0; i < 2; ++i) L.lE_[i] = L.lH_[i] = L.lu_[i] = L.lpE_[i] = off;
   const int nsteps = steps_of(p);
   if (p.ckpt_every > 0 && nsteps > 0) {
     L.S = p.ckpt_every < nsteps ? p.ckpt_every : nsteps;
@@ -94,10 +97,12 @@ inline Layout make_layout(const Geom& g, const pjz_fdtd_problem& p) {
     // D^b .. D^(b+S-1) of a segment, plus one slot for the extra step a
     // batched pair runs past an odd step count.
     L.seg = off; off += f3 * (size_t)(L.S + 1);
-    L.lE = off; off += f3;
-    L.lH = off; off += f3;
-    L.lu = off; off += p2;
-    L.lpE = off; off += p2;
+    for (int i = 0; i < 2; ++i) {
+      L.lE_[i] = off; off += f3;
+      L.lH_[i] = off; off += f3;
+      L.lu_[i] = off; off += p2;
+      L.lpE_[i] = off; off += p2;
+    }
     L.accG = off; off += f3;
   }
   L.total = off;
@@ -177,10 +182,19 @@ hipError_t batch2_pairs(const Batch2Plan& plan, int m0, int m1, hipStream_t s);
 hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s);
 hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s);
 
-// Reverse of step n: lE, lH, lu, lpE hold the cotangents after step n on
-// entry and before it on exit; G accumulates lE' D^n (D^n = E^n - E^(n+1)).
+// Reverse of step n: adjoint set `acur` (lE, lH, lu, lpE) holds the
+// cotangents after step n on entry and before it on exit (in place); G
+// accumulates lE' D^n (D^n = E^n - E^(n+1)).
 hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
-                               char* ws, const Layout& L, const float* D, hipStream_t s);
+                               char* ws, const Layout& L, const float* D, int acur,
+                               hipStream_t s);
+
+// The same reverse step as one streaming launch: reads adjoint set `acur`,
+// writes set 1 - acur.
+bool adjoint_stream_supported(const Geom& g);
+hipError_t launch_adjoint_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                                 char* ws, const Layout& L, const float* D, int acur,
+                                 hipStream_t s);
 
 // Batched reverse steps: one launch per half step for every simulation of a
 // batch (blocks prefix-summed over simulations).  Each item's D^n is in its

@@ -161,12 +161,12 @@ __global__ void k_adj_e(AdjArgs a) {
 }
 
 static AdjArgs adj_args(const Geom& g, const pjz_fdtd_problem& p, const float* mask, char* ws,
-                        const Layout& L) {
+                        const Layout& L, int acur) {
   AdjArgs a;
-  a.lE = (float*)(ws + L.lE);
-  a.lH = (float*)(ws + L.lH);
-  a.lu = (float*)(ws + L.lu);
-  a.lpE = (float*)(ws + L.lpE);
+  a.lE = (float*)(ws + L.lE_[acur]);
+  a.lH = (float*)(ws + L.lH_[acur]);
+  a.lu = (float*)(ws + L.lu_[acur]);
+  a.lpE = (float*)(ws + L.lpE_[acur]);
   a.G = (float*)(ws + L.accG);
   a.cb = (const float*)(ws + L.cb);
   a.zt = (const float*)(ws + L.zt);
@@ -178,8 +178,9 @@ static AdjArgs adj_args(const Geom& g, const pjz_fdtd_problem& p, const float* m
 }
 
 hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
-                               char* ws, const Layout& L, const float* D, hipStream_t s) {
-  AdjArgs a = adj_args(g, p, mask, ws, L);
+                               char* ws, const Layout& L, const float* D, int acur,
+                               hipStream_t s) {
+  AdjArgs a = adj_args(g, p, mask, ws, L, acur);
   a.D = D;
   const unsigned nb = nblocks(g.N, 256);
   hipLaunchKernelGGL(k_adj_h, dim3(nb), dim3(256), 0, s, a);
@@ -230,7 +231,7 @@ hipError_t adjoint_batch_prepare(const BatchItem* items, int n, char* meta, AdjB
   long long blocks = 0;
   for (int i = 0; i < n; ++i) {
     const BatchItem& it = items[i];
-    host[i].a = adj_args(it.g, *it.p, it.mask, it.ws, *it.L);
+    host[i].a = adj_args(it.g, *it.p, it.mask, it.ws, *it.L, 0);
     host[i].a.D = it.capture;
     host[i].nsteps = steps_of(*it.p);
     host[i].block0 = (int)blocks;

@@ -1,0 +1,345 @@
+// Streaming reverse step: one fused launch per reversed time step, the
+// adjoint counterpart of the one-step streaming stepper.  The two halves of
+// a reverse step (kernels_adjoint.hip gives them cell by cell) have the
+// forward kernel's dependency shape mirrored:
+//   lH update at (x, y, z) reads lK = cb lE at x+1, y+1, z+1 (like the
+//     forward H update reads E there) and lpsiE at z, z+1;
+//   lE update at (x, y, z) reads the new lH at x-1, y-1, z-1 (like the
+//     forward E update reads H there) and the new psi_H cotangent at z, z-1.
+// So the same sweep works: one wavefront per y row (whole z column, CPL cells
+// per lane, z neighbours by lane shuffles), a workgroup of WY rows plus one
+// ghost row below (its lH feeds row y0's lE update), rows exchanging lK and
+// the new lH through LDS, x swept over a chunk with the new lH of plane x-1
+// carried in registers and plane xa-1 recomputed as a ghost plane.  All of
+// lE, lH and both psi cotangents ping-pong between two sets (ghost cells read
+// values their owners rewrite in the same launch); G is updated in place by
+// the owner only.
+//
+// Per cell and step it reads lE, cb, lH, D, G (+ the psi cotangents in PML
+// cells, the absorber per column) and writes lE, lH, G: 96 B of the 168 B the
+// two cell-by-cell kernels move.
+#include <algorithm>
+
+#include "stream_common.h"
+
+namespace pjz {
+
+struct AdjStreamArgs {
+  const float* lEin;
+  const float* lHin;
+  float* lEout;
+  float* lHout;
+  const float* luin;   // psi_H cotangent pair (x at +0, y at +P)
+  float* luout;
+  const float* lpin;   // psi_E cotangent pair
+  float* lpout;
+  float* G;
+  const float* D;      // E^n - E^(n+1) of the step being reversed
+  const float* cb;
+  const float* zt;
+  const float* mask;
+  int X, Y, Z, lo, hi, NP, N, P;
+  float dt;
+  int LX, nty, ntiles;
+};
+
+// Everything one wave reads for plane x of its row, fetched one plane ahead.
+template <int CPL>
+struct AdjPlane {
+  float le[3][CPL];   // lE at x + 1
+  float cb[3][CPL];   // cb at x + 1
+  float lh[3][CPL];   // lH at x
+  float lu[2][CPL];   // psi_H cotangent at x (PML cells)
+  float lp[2][CPL];   // psi_E cotangent at x (PML cells)
+  float d[3][CPL];    // D at x (owner cells)
+  float g[3][CPL];    // G at x (owner cells)
+  float kup[2][CPL];  // lK x, z at (x, y+1) (top row of the workgroup only)
+  float sig[3];       // absorber at (x, y) (owner cells)
+};
+
+template <int CPL, int WY, bool vec>
+__device__ inline void adj_fetch(AdjPlane<CPL>& P, const AdjStreamArgs& a, int x, int y, int w,
+                                 int zb, bool active, bool owner_cells, const bool (&pml)[CPL],
+                                 const int (&pidx)[CPL]) {
+  const int N = a.N, Y = a.Y, Z = a.Z;
+  const int o = (x * Y + y) * Z;
+  const bool nx = active && x + 1 < a.X;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    load_col<CPL, vec>(P.le[k], a.lEin + (k * N + o + Y * Z), zb, Z, nx);
+    load_col<CPL, vec>(P.cb[k], a.cb + (k * N + o + Y * Z), zb, Z, nx);
+    load_col<CPL, vec>(P.lh[k], a.lHin + (k * N + o), zb, Z, active);
+    load_col<CPL, vec>(P.d[k], a.D + (k * N + o), zb, Z, owner_cells);
+    load_col<CPL, vec>(P.g[k], a.G + (k * N + o), zb, Z, owner_cells);
+  }
+  if (w == WY) {
+    const bool v = active && y + 1 < Y;
+    float e0[CPL], e2[CPL], c0[CPL], c2[CPL];
+    load_col<CPL, vec>(e0, a.lEin + (o + Z), zb, Z, v);
+    load_col<CPL, vec>(e2, a.lEin + (2 * N + o + Z), zb, Z, v);
+    load_col<CPL, vec>(c0, a.cb + (o + Z), zb, Z, v);
+    load_col<CPL, vec>(c2, a.cb + (2 * N + o + Z), zb, Z, v);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      P.kup[0][c] = c0[c] * e0[c];
+      P.kup[1][c] = c2[c] * e2[c];
+    }
+  }
+  const int pb = (x * Y + y) * a.NP;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const bool ph = pml[c] && active;
+    const int pi = pb + pidx[c];
+    P.lu[0][c] = ph ? a.luin[pi] : 0.0f;
+    P.lu[1][c] = ph ? a.luin[a.P + pi] : 0.0f;
+    P.lp[0][c] = ph ? a.lpin[pi] : 0.0f;
+    P.lp[1][c] = ph ? a.lpin[a.P + pi] : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    P.sig[k] = owner_cells ? a.mask[(k * a.X + x) * Y + y] : 0.0f;
+}
+
+template <int CPL, int WY, bool vec>
+__global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream(AdjStreamArgs a) {
+  const int X = a.X, Y = a.Y, Z = a.Z, N = a.N;
+  const float dt = a.dt;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // 0 = ghost row
+  const int zb = lane * CPL;
+  int tile = blockIdx.x;
+  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
+  const int ty = tile % a.nty, tx = tile / a.nty;
+  const int y0 = ty * WY;
+  const int y = y0 - 1 + w;
+  const bool row_valid = y >= 0 && y < Y;
+  const int yc = min(max(y, 0), Y - 1);
+  const bool owner = w >= 1 && row_valid;
+  const int xa = tx * a.LX;
+  const int xb = min(xa + a.LX, X);
+
+  __shared__ float sK[WY + 1][2][64 * CPL];  // lK x, z of plane x, per row
+  __shared__ float sH[WY + 1][2][64 * CPL];  // new lH x, z of plane x, per row
+
+  // z coefficients of the lane's cells and of their z+1 / z-1 neighbours.
+  float ikE[CPL], ikEp[CPL], bE[CPL], cE[CPL], ikH[CPL], ikHm[CPL], bH[CPL], cH[CPL];
+  bool pml[CPL];
+  int pidx[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const int z = zb + c;
+    const int zz = min(z, Z - 1);
+    ikE[c] = a.zt[IKE * Z + zz];
+    ikEp[c] = z + 1 < Z ? a.zt[IKE * Z + z + 1] : 0.0f;
+    bE[c] = a.zt[BE * Z + zz];
+    cE[c] = a.zt[CE * Z + zz];
+    ikH[c] = a.zt[IKH * Z + zz];
+    ikHm[c] = z >= 1 && z - 1 < Z ? a.zt[IKH * Z + z - 1] : 0.0f;
+    bH[c] = a.zt[BH * Z + zz];
+    cH[c] = a.zt[CH * Z + zz];
+    pml[c] = z < Z && (z < a.lo || z >= Z - a.hi);
+    pidx[c] = z < a.lo ? z : z - (Z - a.hi) + a.lo;
+  }
+
+  // Carried: lE and cb at x (this row), new lH y, z at x-1.
+  float le0[3][CPL], cb0[3][CPL], hyp[CPL], hzp[CPL];
+  const int xs = max(xa - 1, 0);
+  {
+    const int o = (xs * Y + yc) * Z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      load_col<CPL, vec>(le0[k], a.lEin + (k * N + o), zb, Z, row_valid);
+      load_col<CPL, vec>(cb0[k], a.cb + (k * N + o), zb, Z, row_valid);
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) hyp[c] = hzp[c] = 0.0f;
+  }
+  AdjPlane<CPL> cur, nxt;
+  adj_fetch<CPL, WY, vec>(cur, a, xs, yc, w, zb, row_valid, owner && xs >= xa, pml, pidx);
+
+  for (int x = xs; x < xb; ++x) {
+    const bool ghost_plane = x < xa;
+    const bool store = owner && !ghost_plane;
+    const int o = (x * Y + y) * Z;
+    adj_fetch<CPL, WY, vec>(nxt, a, x + 1, yc, w, zb, row_valid && x + 1 < xb,
+                            owner && x + 1 < xb, pml, pidx);
+    float K[3][CPL];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) K[k][c] = cb0[k][c] * le0[k][c];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      sK[w][0][zb + c] = K[0][c];
+      sK[w][1][zb + c] = K[2][c];
+    }
+    lds_barrier();
+    float kyp[2][CPL];  // lK x, z of the row above
+    if (w < WY) {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        kyp[0][c] = sK[w + 1][0][zb + c];
+        kyp[1][c] = sK[w + 1][1][zb + c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        kyp[0][c] = cur.kup[0][c];
+        kyp[1][c] = cur.kup[1][c];
+      }
+    }
+
+    // ---- lH update at plane x, row y
+    float kxz[CPL], kyz[CPL], vx[CPL], vy[CPL], vxz[CPL], vyz[CPL], h[3][CPL], lun[2][CPL];
+    z_next<CPL>(kxz, K[0], zb, Z);
+    z_next<CPL>(kyz, K[1], zb, Z);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      vx[c] = pml[c] ? cE[c] * (cur.lp[0][c] - K[0][c]) : 0.0f;
+      vy[c] = pml[c] ? cE[c] * (cur.lp[1][c] + K[1][c]) : 0.0f;
+    }
+    z_next<CPL>(vxz, vx, zb, Z);
+    z_next<CPL>(vyz, vy, zb, Z);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      // lK y, z at x+1 from the prefetched plane.
+      const float kxp_y = cur.cb[1][c] * cur.le[1][c];
+      const float kxp_z = cur.cb[2][c] * cur.le[2][c];
+      h[0][c] = cur.lh[0][c] - (ikEp[c] * kyz[c] - ikE[c] * K[1][c]) + (kyp[1][c] - K[2][c]) -
+                (vyz[c] - vy[c]);
+      h[1][c] = cur.lh[1][c] + (ikEp[c] * kxz[c] - ikE[c] * K[0][c]) - (kxp_z - K[2][c]) -
+                (vxz[c] - vx[c]);
+      h[2][c] = cur.lh[2][c] - (kyp[0][c] - K[0][c]) + (kxp_y - K[1][c]);
+      // Rows outside the domain hold no cells.
+      if (!row_valid) h[0][c] = h[1][c] = h[2][c] = 0.0f;
+      lun[0][c] = pml[c] ? bH[c] * cur.lu[0][c] + dt * h[0][c] : 0.0f;
+      lun[1][c] = pml[c] ? bH[c] * cur.lu[1][c] - dt * h[1][c] : 0.0f;
+    }
+    if (store) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) store_col<CPL, vec>(a.lHout + (k * N + o), h[k], zb, Z);
+      const int pb = (x * Y + y) * a.NP;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        if (pml[c]) {
+          a.luout[pb + pidx[c]] = lun[0][c];
+          a.luout[a.P + pb + pidx[c]] = lun[1][c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      sH[w][0][zb + c] = h[0][c];
+      sH[w][1][zb + c] = h[2][c];
+    }
+    lds_barrier();
+
+    // ---- lE update at plane x, row y (owner rows, chunk planes)
+    if (store) {
+      float hxz[CPL], hyz[CPL], ux[CPL], uy[CPL], uxz[CPL], uyz[CPL];
+      z_prev<CPL>(hxz, h[0], lane);
+      z_prev<CPL>(hyz, h[1], lane);
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        ux[c] = pml[c] ? cH[c] * lun[0][c] : 0.0f;
+        uy[c] = pml[c] ? cH[c] * lun[1][c] : 0.0f;
+      }
+      z_prev<CPL>(uxz, ux, lane);
+      z_prev<CPL>(uyz, uy, lane);
+      float len[3][CPL], gn[3][CPL];
+      const int pb = (x * Y + y) * a.NP;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        const float hx_ym = sH[w - 1][0][zb + c];
+        const float hz_ym = sH[w - 1][1][zb + c];
+        const float gx = dt * ((ikH[c] * h[1][c] - ikHm[c] * hyz[c]) - (h[2][c] - hz_ym)) -
+                         (uy[c] - uyz[c]);
+        const float gy = dt * (-(ikH[c] * h[0][c] - ikHm[c] * hxz[c]) + (h[2][c] - hzp[c])) -
+                         (ux[c] - uxz[c]);
+        const float gz = dt * ((h[0][c] - hx_ym) - (h[1][c] - hyp[c]));
+        const float gs[3] = {gx, gy, gz};
+        float lk[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float le = le0[k][c];
+          gn[k][c] = cur.g[k][c] + le * cur.d[k][c];
+          lk[k] = cb0[k][c] * le;
+          len[k][c] = (1.0f - cur.sig[k] * cb0[k][c]) * le + gs[k];
+        }
+        if (pml[c]) {
+          a.lpout[pb + pidx[c]] = bE[c] * (cur.lp[0][c] - lk[0]);
+          a.lpout[a.P + pb + pidx[c]] = bE[c] * (cur.lp[1][c] + lk[1]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        store_col<CPL, vec>(a.lEout + (k * N + o), len[k], zb, Z);
+        store_col<CPL, vec>(a.G + (k * N + o), gn[k], zb, Z);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      hyp[c] = h[1][c];
+      hzp[c] = h[2][c];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        le0[k][c] = cur.le[k][c];
+        cb0[k][c] = cur.cb[k][c];
+      }
+    }
+    cur = nxt;
+  }
+}
+
+// Whether the streaming reverse step handles this problem.
+bool adjoint_stream_supported(const Geom& g) {
+  return (g.Z <= 64) || (g.Z <= 128 && g.Z % 2 == 0) || (g.Z <= 256 && g.Z % 4 == 0);
+}
+
+hipError_t launch_adjoint_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                                 char* ws, const Layout& L, const float* D, int acur,
+                                 hipStream_t s) {
+  if (!adjoint_stream_supported(g)) return hipErrorInvalidValue;
+  AdjStreamArgs a;
+  const size_t in = acur ? 1 : 0, out = 1 - in;
+  a.lEin = (const float*)(ws + L.lE_[in]);
+  a.lHin = (const float*)(ws + L.lH_[in]);
+  a.luin = (const float*)(ws + L.lu_[in]);
+  a.lpin = (const float*)(ws + L.lpE_[in]);
+  a.lEout = (float*)(ws + L.lE_[out]);
+  a.lHout = (float*)(ws + L.lH_[out]);
+  a.luout = (float*)(ws + L.lu_[out]);
+  a.lpout = (float*)(ws + L.lpE_[out]);
+  a.G = (float*)(ws + L.accG);
+  a.D = D;
+  a.cb = (const float*)(ws + L.cb);
+  a.zt = (const float*)(ws + L.zt);
+  a.mask = mask;
+  a.X = g.X;
+  a.Y = g.Y;
+  a.Z = g.Z;
+  a.lo = g.lo;
+  a.hi = g.hi;
+  a.NP = g.NP;
+  a.N = (int)g.N;
+  a.P = g.X * g.Y * g.NP;
+  a.dt = p.dt;
+  const int cpl = g.Z <= 64 ? 1 : (g.Z <= 128 ? 2 : 4);
+  const int WY = cpl == 1 ? 4 : (cpl == 2 ? 8 : 4);
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  a.nty = (g.Y + WY - 1) / WY;
+  const int ntx = std::max(1, (cus + a.nty - 1) / a.nty);
+  a.LX = std::max(8, (g.X + ntx - 1) / ntx);
+  a.ntiles = a.nty * ((g.X + a.LX - 1) / a.LX);
+  if (cpl == 1)
+    hipLaunchKernelGGL((k_adj_stream<1, 4, true>), dim3(a.ntiles), dim3(64 * 5), 0, s, a);
+  else if (cpl == 2)
+    hipLaunchKernelGGL((k_adj_stream<2, 8, true>), dim3(a.ntiles), dim3(64 * 9), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_adj_stream<4, 4, true>), dim3(a.ntiles), dim3(64 * 5), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace pjz
