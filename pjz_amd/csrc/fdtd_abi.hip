@@ -276,7 +276,7 @@ int batch_plan(const pjz_fdtd_problem* probs, int n, BatchPlanHost& bp) {
   for (int i = 0; i < n; ++i)
     items[i] = {geom_of(probs[i]), &probs[i], nullptr, {}, {}, nullptr, &bp.Ls[i], nullptr};
   bp.adj_off = off + pjz::align256(pjz::batch2_meta_bytes(items.data(), n));
-  bp.total = bp.adj_off + (bp.S > 0 ? pjz::adjoint_batch_meta_bytes(n) : 0);
+  bp.total = bp.adj_off + (bp.S > 0 ? pjz::adjoint_batch_meta_bytes(n, bp.S) : 0);
   return PJZ_FDTD_OK;
 }
 
@@ -399,7 +399,8 @@ int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n, const float* c
   pjz::Batch2Plan plan;
   pjz::AdjBatchPlan aplan;
   hipError_t e = pjz::batch2_prepare(items.data(), n, base + bp.meta_off, plan, s);
-  if (e == hipSuccess) e = pjz::adjoint_batch_prepare(items.data(), n, base + bp.adj_off, aplan, s);
+  if (e == hipSuccess)
+    e = pjz::adjoint_batch_prepare(items.data(), n, bp.S, base + bp.adj_off, aplan, s);
   const int last_seg = (2 * plan.npairs - 1) / bp.S;
   for (int k = last_seg; k >= 0 && e == hipSuccess; --k) {
     const int b = k * bp.S;
@@ -415,14 +416,17 @@ int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n, const float* c
     }
     if (e == hipSuccess) e = pjz::batch2_capture_from(plan, b, s);
     if (e == hipSuccess) e = pjz::batch2_pairs(plan, b / 2, (end + 1) / 2, s);
-    // Reverse steps end-1 .. b of the whole batch.
+    // Reverse steps end-1 .. b of the whole batch.  At reversed step m a
+    // simulation's cotangents sit in adjoint set (nsteps - 1 - m) & 1.
+    if (e == hipSuccess) e = pjz::adjoint_batch_begin_segment(aplan, s);
     for (int m = end - 1; m >= b && e == hipSuccess; --m) {
       for (int i = 0; i < n && e == hipSuccess; ++i) {
         const pjz_fdtd_problem& q = probs[i];
-        if (m < pjz::steps_of(q) && m >= q.out_start && m < q.out_stop &&
-            (m - q.out_start) % q.out_step == 0) {
+        const int ns = pjz::steps_of(q);
+        if (m < ns && m >= q.out_start && m < q.out_stop && (m - q.out_start) % q.out_step == 0) {
           const long long gstride = 3LL * q.ex * q.ey * q.ez;
-          e = pjz::launch_inject(items[i].g, q, (float*)(items[i].ws + bp.Ls[i].lE_[0]),
+          e = pjz::launch_inject(items[i].g, q,
+                                 (float*)(items[i].ws + bp.Ls[i].lE_[(ns - 1 - m) & 1]),
                                  grad_out[i] + gstride * ((m - q.out_start) / q.out_step), s);
         }
       }

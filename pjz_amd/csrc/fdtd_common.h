@@ -196,17 +196,23 @@ hipError_t launch_adjoint_stream(const Geom& g, const pjz_fdtd_problem& p, const
                                  char* ws, const Layout& L, const float* D, int acur,
                                  hipStream_t s);
 
-// Batched reverse steps: one launch per half step for every simulation of a
-// batch (blocks prefix-summed over simulations).  Each item's D^n is in its
-// `capture` buffer at slot n - seg_begin; a simulation skips steps at or past
-// its last.
+// Batched reverse steps: one persistent streaming launch per cells-per-lane
+// class for every simulation of a batch, (simulation, tile) items taken from
+// a device queue.  Each item's D^n is in its `capture` buffer at slot
+// n - seg_begin; a simulation with nsteps steps reads adjoint set
+// (nsteps - 1 - n) & 1 at reversed step n (sets alternate per reversed step,
+// starting from set 0) and skips steps at or past its last.  S bounds the
+// steps of a segment; adjoint_batch_begin_segment clears the queue counters.
 struct AdjBatchPlan {
-  int n, blocks;
-  char* meta;  // device table of per-simulation arguments and first blocks
+  int n, S, cus;
+  int count[2], base[2], total[2];
+  char* meta;
+  size_t args_bytes, ts_bytes;
 };
-size_t adjoint_batch_meta_bytes(int n);
-hipError_t adjoint_batch_prepare(const BatchItem* items, int n, char* meta, AdjBatchPlan& plan,
-                                 hipStream_t s);
+size_t adjoint_batch_meta_bytes(int n, int S);
+hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, char* meta,
+                                 AdjBatchPlan& plan, hipStream_t s);
+hipError_t adjoint_batch_begin_segment(const AdjBatchPlan& plan, hipStream_t s);
 hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int step, int seg_begin,
                                      hipStream_t s);
 

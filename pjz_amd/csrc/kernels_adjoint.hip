@@ -16,8 +16,6 @@
 // dL/deps = (cb/dt) (A - B) = (cb/dt) G.  Accumulating G from the per-step
 // change of E halves the accumulator traffic and avoids the cancellation of
 // A - B.
-#include <vector>
-
 #include "fdtd_common.h"
 
 namespace pjz {
@@ -185,75 +183,6 @@ hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const f
   const unsigned nb = nblocks(g.N, 256);
   hipLaunchKernelGGL(k_adj_h, dim3(nb), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_adj_e, dim3(nb), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------- batches
-// One table entry per simulation: its arguments (D holds the capture base),
-// its step count and its first block in the batch launch.
-struct AdjItem {
-  AdjArgs a;
-  int nsteps;
-  int block0;
-};
-
-// Simulation of block b: the last entry with block0 <= b.
-__device__ inline int adj_item_of(const AdjItem* items, int n, int b) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].block0 <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-__global__ void k_adj_h_batch(const AdjItem* __restrict__ items, int n, int step) {
-  const AdjItem& it = items[adj_item_of(items, n, blockIdx.x)];
-  if (step >= it.nsteps) return;
-  const long long idx = (blockIdx.x - it.block0) * (long long)blockDim.x + threadIdx.x;
-  if (idx < it.a.g.N) adj_h_cell(it.a, idx);
-}
-
-__global__ void k_adj_e_batch(const AdjItem* __restrict__ items, int n, int step, int seg_begin) {
-  const AdjItem& it = items[adj_item_of(items, n, blockIdx.x)];
-  if (step >= it.nsteps) return;
-  const long long idx = (blockIdx.x - it.block0) * (long long)blockDim.x + threadIdx.x;
-  if (idx >= it.a.g.N) return;
-  adj_e_cell(it.a, idx, it.a.D + (long long)(step - seg_begin) * 3 * it.a.g.N);
-}
-
-size_t adjoint_batch_meta_bytes(int n) { return align256(sizeof(AdjItem) * (size_t)n); }
-
-hipError_t adjoint_batch_prepare(const BatchItem* items, int n, char* meta, AdjBatchPlan& plan,
-                                 hipStream_t s) {
-  std::vector<AdjItem> host(n);
-  long long blocks = 0;
-  for (int i = 0; i < n; ++i) {
-    const BatchItem& it = items[i];
-    host[i].a = adj_args(it.g, *it.p, it.mask, it.ws, *it.L, 0);
-    host[i].a.D = it.capture;
-    host[i].nsteps = steps_of(*it.p);
-    host[i].block0 = (int)blocks;
-    blocks += nblocks(it.g.N, 256);
-  }
-  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-  plan.n = n;
-  plan.blocks = (int)blocks;
-  plan.meta = meta;
-  hipError_t e = hipMemcpyAsync(meta, host.data(), sizeof(AdjItem) * (size_t)n,
-                                hipMemcpyHostToDevice, s);
-  if (e != hipSuccess) return e;
-  // The host table must outlive the copy.
-  return hipStreamSynchronize(s);
-}
-
-hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int step, int seg_begin,
-                                     hipStream_t s) {
-  const AdjItem* items = (const AdjItem*)plan.meta;
-  hipLaunchKernelGGL(k_adj_h_batch, dim3(plan.blocks), dim3(256), 0, s, items, plan.n, step);
-  hipLaunchKernelGGL(k_adj_e_batch, dim3(plan.blocks), dim3(256), 0, s, items, plan.n, step,
-                     seg_begin);
   return hipGetLastError();
 }
 

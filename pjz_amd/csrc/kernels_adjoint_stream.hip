@@ -19,6 +19,7 @@
 // cells, the absorber per column) and writes lE, lH, G: 96 B of the 168 B the
 // two cell-by-cell kernels move.
 #include <algorithm>
+#include <vector>
 
 #include "stream_common.h"
 
@@ -41,6 +42,13 @@ struct AdjStreamArgs {
   int X, Y, Z, lo, hi, NP, N, P;
   float dt;
   int LX, nty, ntiles;
+  int nsteps;  // steps of the simulation (batched runs)
+};
+
+template <int CPL, int WY>
+struct AdjLds {
+  float sK[WY + 1][2][64 * CPL];  // lK x, z of plane x, per row
+  float sH[WY + 1][2][64 * CPL];  // new lH x, z of plane x, per row
 };
 
 // Everything one wave reads for plane x of its row, fetched one plane ahead.
@@ -100,15 +108,14 @@ __device__ inline void adj_fetch(AdjPlane<CPL>& P, const AdjStreamArgs& a, int x
     P.sig[k] = owner_cells ? a.mask[(k * a.X + x) * Y + y] : 0.0f;
 }
 
+// One tile (WY rows x LX planes) of one reversed step.
 template <int CPL, int WY, bool vec>
-__global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream(AdjStreamArgs a) {
+__device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLds<CPL, WY>& S) {
   const int X = a.X, Y = a.Y, Z = a.Z, N = a.N;
   const float dt = a.dt;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // 0 = ghost row
   const int zb = lane * CPL;
-  int tile = blockIdx.x;
-  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
   const int ty = tile % a.nty, tx = tile / a.nty;
   const int y0 = ty * WY;
   const int y = y0 - 1 + w;
@@ -118,8 +125,8 @@ __global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream(AdjStreamArgs a) {
   const int xa = tx * a.LX;
   const int xb = min(xa + a.LX, X);
 
-  __shared__ float sK[WY + 1][2][64 * CPL];  // lK x, z of plane x, per row
-  __shared__ float sH[WY + 1][2][64 * CPL];  // new lH x, z of plane x, per row
+  auto& sK = S.sK;
+  auto& sH = S.sH;
 
   // z coefficients of the lane's cells and of their z+1 / z-1 neighbours.
   float ikE[CPL], ikEp[CPL], bE[CPL], cE[CPL], ikH[CPL], ikHm[CPL], bH[CPL], cH[CPL];
@@ -291,15 +298,65 @@ __global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream(AdjStreamArgs a) {
   }
 }
 
+template <int CPL, int WY, bool vec>
+__global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream(AdjStreamArgs a) {
+  __shared__ AdjLds<CPL, WY> S;
+  // Tiles adjacent in y (sharing ghost rows) go to the same XCD.
+  int tile = blockIdx.x;
+  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
+  adj_tile<CPL, WY, vec>(a, tile, S);
+}
+
+// Batched reverse step: persistent workgroups take (simulation, tile) items
+// from a device counter, as the batched two-step forward kernel does.  args
+// holds two views per simulation (which adjoint set is read); a simulation
+// with nsteps steps has reversed nsteps-1-step steps before this one, which
+// fixes the set it reads.  D of the step sits at slot step - seg_begin of the
+// view's D base.
+template <int CPL, int WY, bool vec>
+__global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream_batch(
+    const AdjStreamArgs* __restrict__ args, const int* __restrict__ tile_start, int nsims,
+    int total, int* __restrict__ counter, int step, int seg_begin) {
+  __shared__ AdjLds<CPL, WY> S;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (t >= total) break;
+    int lo = 0, hi = nsims - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const int ns = args[2 * lo].nsteps;
+    if (step >= ns) continue;
+    AdjStreamArgs a = args[2 * lo + ((ns - 1 - step) & 1)];
+    a.D += (long long)(step - seg_begin) * 3 * a.N;
+    adj_tile<CPL, WY, vec>(a, t - tile_start[lo], S);
+  }
+}
+
 // Whether the streaming reverse step handles this problem.
 bool adjoint_stream_supported(const Geom& g) {
   return (g.Z <= 64) || (g.Z <= 128 && g.Z % 2 == 0) || (g.Z <= 256 && g.Z % 4 == 0);
 }
 
-hipError_t launch_adjoint_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
-                                 char* ws, const Layout& L, const float* D, int acur,
-                                 hipStream_t s) {
-  if (!adjoint_stream_supported(g)) return hipErrorInvalidValue;
+static int device_cus() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return cus;
+}
+
+static int rows_per_tile(const Geom& g) { return g.Z > 64 && g.Z <= 128 ? 8 : 4; }
+
+// Arguments of one reversed step reading adjoint set acur; LX = 0 picks x
+// chunks that give about one workgroup per CU.
+static AdjStreamArgs adj_stream_args(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                                     char* ws, const Layout& L, const float* D, int acur, int LX) {
   AdjStreamArgs a;
   const size_t in = acur ? 1 : 0, out = 1 - in;
   a.lEin = (const float*)(ws + L.lE_[in]);
@@ -324,21 +381,110 @@ hipError_t launch_adjoint_stream(const Geom& g, const pjz_fdtd_problem& p, const
   a.N = (int)g.N;
   a.P = g.X * g.Y * g.NP;
   a.dt = p.dt;
-  const int cpl = g.Z <= 64 ? 1 : (g.Z <= 128 ? 2 : 4);
-  const int WY = cpl == 1 ? 4 : (cpl == 2 ? 8 : 4);
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess)
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int WY = rows_per_tile(g);
   a.nty = (g.Y + WY - 1) / WY;
-  const int ntx = std::max(1, (cus + a.nty - 1) / a.nty);
-  a.LX = std::max(8, (g.X + ntx - 1) / ntx);
+  if (LX <= 0) {
+    const int ntx = std::max(1, (device_cus() + a.nty - 1) / a.nty);
+    LX = std::max(8, (g.X + ntx - 1) / ntx);
+  }
+  a.LX = LX;
   a.ntiles = a.nty * ((g.X + a.LX - 1) / a.LX);
+  a.nsteps = steps_of(p);
+  return a;
+}
+
+hipError_t launch_adjoint_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                                 char* ws, const Layout& L, const float* D, int acur,
+                                 hipStream_t s) {
+  if (!adjoint_stream_supported(g)) return hipErrorInvalidValue;
+  AdjStreamArgs a = adj_stream_args(g, p, mask, ws, L, D, acur, 0);
+  const int cpl = g.Z <= 64 ? 1 : (g.Z <= 128 ? 2 : 4);
   if (cpl == 1)
     hipLaunchKernelGGL((k_adj_stream<1, 4, true>), dim3(a.ntiles), dim3(64 * 5), 0, s, a);
   else if (cpl == 2)
     hipLaunchKernelGGL((k_adj_stream<2, 8, true>), dim3(a.ntiles), dim3(64 * 9), 0, s, a);
   else
     hipLaunchKernelGGL((k_adj_stream<4, 4, true>), dim3(a.ntiles), dim3(64 * 5), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- batches
+// Meta layout: two argument views per simulation (cells-per-lane class
+// order) | tile prefix sums (n_c + 1 per class) | queue counters (2 per
+// reversed step of a segment).
+static constexpr int kAdjBatchLX = 128;
+
+size_t adjoint_batch_meta_bytes(int n, int S) {
+  return align256(sizeof(AdjStreamArgs) * 2 * (size_t)n) + align256(sizeof(int) * (size_t)(n + 2)) +
+         align256(sizeof(int) * 2 * (size_t)std::max(S, 1));
+}
+
+hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, char* meta,
+                                 AdjBatchPlan& plan, hipStream_t s) {
+  std::vector<int> cls[2];
+  for (int i = 0; i < n; ++i) {
+    if (!adjoint_stream_supported(items[i].g) || items[i].g.Z > 128) return hipErrorInvalidValue;
+    cls[items[i].g.Z <= 64 ? 0 : 1].push_back(i);
+  }
+  plan.n = n;
+  plan.S = std::max(S, 1);
+  plan.meta = meta;
+  plan.cus = device_cus();
+  plan.args_bytes = align256(sizeof(AdjStreamArgs) * 2 * (size_t)n);
+  plan.ts_bytes = align256(sizeof(int) * (size_t)(n + 2));
+  std::vector<char> host(plan.args_bytes + plan.ts_bytes, 0);
+  AdjStreamArgs* hargs = (AdjStreamArgs*)host.data();
+  int* hts = (int*)(host.data() + plan.args_bytes);
+  int off = 0;
+  for (int c = 0; c < 2; ++c) {
+    plan.base[c] = off;
+    plan.count[c] = (int)cls[c].size();
+    int t = 0;
+    for (size_t k = 0; k < cls[c].size(); ++k) {
+      const BatchItem& it = items[cls[c][k]];
+      const int LX = std::min(kAdjBatchLX, it.g.X);
+      for (int v = 0; v < 2; ++v)
+        hargs[2 * (off + k) + v] =
+            adj_stream_args(it.g, *it.p, it.mask, it.ws, *it.L, it.capture, v, LX);
+      hts[off + c + k] = t;
+      t += hargs[2 * (off + k)].ntiles;
+    }
+    hts[off + c + cls[c].size()] = t;
+    plan.total[c] = t;
+    off += (int)cls[c].size();
+  }
+  hipError_t e = hipMemcpyAsync(meta, host.data(), host.size(), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  // The host image must outlive the copy.
+  return hipStreamSynchronize(s);
+}
+
+hipError_t adjoint_batch_begin_segment(const AdjBatchPlan& plan, hipStream_t s) {
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes);
+  return launch_zero((float*)counters, 2 * (size_t)plan.S, s);
+}
+
+hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int step, int seg_begin,
+                                     hipStream_t s) {
+  const int slot = step - seg_begin;
+  if (slot < 0 || slot >= plan.S) return hipErrorInvalidValue;
+  const AdjStreamArgs* dargs = (const AdjStreamArgs*)plan.meta;
+  const int* dts = (const int*)(plan.meta + plan.args_bytes);
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes);
+  for (int c = 0; c < 2; ++c) {
+    const int nc = plan.count[c];
+    if (nc == 0) continue;
+    const int grid = std::min(plan.total[c], plan.cus);
+    const AdjStreamArgs* a = dargs + 2 * plan.base[c];
+    const int* ts = dts + plan.base[c] + c;
+    int* ctr = counters + 2 * slot + c;
+    if (c == 0)
+      hipLaunchKernelGGL((k_adj_stream_batch<1, 4, true>), dim3(grid), dim3(64 * 5), 0, s, a, ts,
+                         nc, plan.total[c], ctr, step, seg_begin);
+    else
+      hipLaunchKernelGGL((k_adj_stream_batch<2, 8, true>), dim3(grid), dim3(64 * 9), 0, s, a, ts,
+                         nc, plan.total[c], ctr, step, seg_begin);
+  }
   return hipGetLastError();
 }
 
