@@ -58,11 +58,13 @@ def load():
   with _lock:
     if _lib is not None:
       return _lib
-    if not os.path.exists(LIB_PATH):
+    # PJZ_FDTD_LIB selects another build of the same library (A/B timing).
+    path = os.environ.get("PJZ_FDTD_LIB") or LIB_PATH
+    if not os.path.exists(path):
       raise RuntimeError(
-          f"{LIB_PATH} is missing: build it with `python -m pjz_amd.build` "
+          f"{path} is missing: build it with `python -m pjz_amd.build` "
           "(the FDTD kernels have no CPU fallback)")
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     P = ctypes.POINTER(Problem)
     vp, sz = ctypes.c_void_p, ctypes.c_size_t
     lib.pjz_fdtd_abi_version.restype = ctypes.c_int

@@ -23,7 +23,7 @@ def amplitudes(beta, vals, x):
   [exp(-i beta x); exp(i beta x)] via its pseudo-inverse, contracted as
   ``einsum("...ji,...j->...i", pinv(a), vals)``.
   """
-  bx = beta[:, None] * x[None, :].to(beta.dtype)
+  bx = beta[:, None] * x[None, :].to(device=beta.device, dtype=beta.dtype)
   a = torch.stack([torch.exp(-1j * bx), torch.exp(1j * bx)], dim=1)
   return torch.einsum("...ji,...j->...i", torch.linalg.pinv(a), vals)
 

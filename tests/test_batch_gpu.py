@@ -63,7 +63,11 @@ def test_batch64_builder_small():
   outs = F.fdtdz_batch(calls)
   for kw, out in zip(calls, outs):
     single = F.fdtdz(**dict(kw, launch_params=dict(kernel="stream2")))
-    assert torch.equal(out, single)
+    if workloads.time_steps(kw) % 2 == 0:
+      assert torch.equal(out, single)
+    else:  # odd last step: level 1 of a pair here, the one-step kernel alone
+      diff = float(torch.linalg.vector_norm(out - single))
+      assert diff <= 1e-6 * float(torch.linalg.vector_norm(single)) + 1e-30
 
 
 def test_batch_rejects_unsupported_height():
