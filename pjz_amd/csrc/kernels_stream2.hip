@@ -29,8 +29,10 @@
 namespace pjz {
 
 // Data touched only by launches that write snapshots or capture E.  It lives
-// in device memory and is read inside a non-inlined store routine, so the
-// stepping loop does not keep it in (scarce) scalar registers.
+// in device memory and is read only inside the (uniform) branches that store,
+// so the stepping loop does not keep it in (scarce) scalar registers.  (A
+// non-inlined store routine is worse: taking the launch arguments by
+// reference moves them to scratch memory, which made the kernel 3.8x slower.)
 struct Stream2Rare {
   float* out;     // snapshots (nn, 3, wex, wey, wez) or null
   float* cap;     // E^m - E^(m+1) of the whole domain at cap + (m - cap_begin) * 3N
@@ -237,7 +239,7 @@ __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y
 // Writes the snapshot of E after `step` (en) and/or the capture of its change
 // over the step (eo - en) for one row at plane x.
 template <int CPL, bool vec>
-__device__ __noinline__ void store_rare(const Stream2Args& a, bool snap, bool cap, int step, int x,
+__device__ __forceinline__ void store_rare(const Stream2Args& a, bool snap, bool cap, int step, int x,
                                         int y, int zb, const float (&en)[3][CPL],
                                         const float (&eo)[3][CPL]) {
   const Stream2Rare* r = a.rare;
