@@ -47,7 +47,7 @@ def overlap(mode, beta, pos, is_fwd, output):
   vals = torch.stack(
       [torch.sum(m * transverse_slice(output, p, axis).to(torch.complex128), dim=(-4, -3, -2, -1))
        for p in sample_at], dim=-1)
-  return amplitudes(beta, vals, x.to(vals.device))
+  return amplitudes(beta.to(vals.device), vals, x.to(vals.device))
 
 
 def _impl(epsilon, omega, modes, betas, pos, is_fwd, sim_params):
@@ -96,7 +96,8 @@ def scatter(epsilon, omega, modes: Sequence[torch.Tensor], betas: Sequence[torch
   with respect to ``epsilon`` through pjz's reciprocity gradient.
   """
   modes = tuple(modes)
-  betas = tuple(torch.as_tensor(b) for b in betas)
+  dev = epsilon.device if isinstance(epsilon, torch.Tensor) else None
+  betas = tuple(torch.as_tensor(b, device=dev) for b in betas)
   out = _Scatter.apply(epsilon, omega, modes, betas, tuple(pos), tuple(is_fwd), sim_params)
   n = len(modes)
   return [[out[i, j] for j in range(n)] for i in range(n)]
