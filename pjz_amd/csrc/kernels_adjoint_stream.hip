@@ -45,6 +45,7 @@ struct AdjStreamArgs {
   int nsteps;  // steps of the simulation (batched runs)
 };
 
+// Row exchange buffers (cell c of lane l at [c * 64 + l]: conflict-free banks).
 template <int CPL, int WY>
 struct AdjLds {
   float sK[WY + 1][2][64 * CPL];  // lK x, z of plane x, per row
@@ -177,16 +178,16 @@ __device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLd
       for (int c = 0; c < CPL; ++c) K[k][c] = cb0[k][c] * le0[k][c];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      sK[w][0][zb + c] = K[0][c];
-      sK[w][1][zb + c] = K[2][c];
+      sK[w][0][c * 64 + lane] = K[0][c];
+      sK[w][1][c * 64 + lane] = K[2][c];
     }
     lds_barrier();
     float kyp[2][CPL];  // lK x, z of the row above
     if (w < WY) {
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        kyp[0][c] = sK[w + 1][0][zb + c];
-        kyp[1][c] = sK[w + 1][1][zb + c];
+        kyp[0][c] = sK[w + 1][0][c * 64 + lane];
+        kyp[1][c] = sK[w + 1][1][c * 64 + lane];
       }
     } else {
 #pragma unroll
@@ -236,8 +237,8 @@ __device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLd
     }
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      sH[w][0][zb + c] = h[0][c];
-      sH[w][1][zb + c] = h[2][c];
+      sH[w][0][c * 64 + lane] = h[0][c];
+      sH[w][1][c * 64 + lane] = h[2][c];
     }
     lds_barrier();
 
@@ -257,8 +258,8 @@ __device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLd
       const int pb = (x * Y + y) * a.NP;
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        const float hx_ym = sH[w - 1][0][zb + c];
-        const float hz_ym = sH[w - 1][1][zb + c];
+        const float hx_ym = sH[w - 1][0][c * 64 + lane];
+        const float hz_ym = sH[w - 1][1][c * 64 + lane];
         const float gx = dt * ((ikH[c] * h[1][c] - ikHm[c] * hyz[c]) - (h[2][c] - hz_ym)) -
                          (uy[c] - uyz[c]);
         const float gy = dt * (-(ikH[c] * h[0][c] - ikHm[c] * hxz[c]) + (h[2][c] - hzp[c])) -

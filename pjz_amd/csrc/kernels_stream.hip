@@ -118,6 +118,7 @@ k_stream(StreamArgs a) {
   const int xa = tx * a.LX;
   const int xb = min(xa + a.LX, X);
 
+  // Row exchange (cell c of lane l at [c * 64 + l]: conflict-free banks).
   __shared__ float sE[WY + 1][2][64 * CPL];  // Ex, Ez of plane x, per row
   __shared__ float sH[WY + 1][2][64 * CPL];  // new Hx, Hz of plane x, per row
 
@@ -160,16 +161,16 @@ k_stream(StreamArgs a) {
                               owner && x + 1 < xb, pml, pidx, pstride);
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      sE[w][0][zb + c] = e0[0][c];
-      sE[w][1][zb + c] = e0[2][c];
+      sE[w][0][c * 64 + lane] = e0[0][c];
+      sE[w][1][c * 64 + lane] = e0[2][c];
     }
     lds_barrier();
     float eup[2][CPL];
     if (w < WY) {
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        eup[0][c] = sE[w + 1][0][zb + c];
-        eup[1][c] = sE[w + 1][1][zb + c];
+        eup[0][c] = sE[w + 1][0][c * 64 + lane];
+        eup[1][c] = sE[w + 1][1][c * 64 + lane];
       }
     } else {
 #pragma unroll
@@ -213,8 +214,8 @@ k_stream(StreamArgs a) {
     }
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      sH[w][0][zb + c] = hn[0][c];
-      sH[w][1][zb + c] = hn[2][c];
+      sH[w][0][c * 64 + lane] = hn[0][c];
+      sH[w][1][c * 64 + lane] = hn[2][c];
     }
     lds_barrier();
 
@@ -226,8 +227,8 @@ k_stream(StreamArgs a) {
       float hdn[2][CPL];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        hdn[0][c] = sH[w - 1][0][zb + c];
-        hdn[1][c] = sH[w - 1][1][zb + c];
+        hdn[0][c] = sH[w - 1][0][c * 64 + lane];
+        hdn[1][c] = sH[w - 1][1][c * 64 + lane];
       }
       float en[3][CPL];
 #pragma unroll

@@ -271,6 +271,8 @@ __device__ __forceinline__ void store_rare(const Stream2Args& a, bool snap, bool
   }
 }
 
+// Row exchange buffers; a row's cell c of lane l sits at [c * 64 + l], so the
+// 64 lanes of an access hit consecutive banks.
 template <int CPL, int WY>
 struct Lds2 {
   float sE1[WY + 3][2][64 * CPL];
@@ -301,16 +303,16 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
   // ---- A: level-1 H at plane x
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sE1[w][0][zb + c] = s.e0[0][c];
-    L.sE1[w][1][zb + c] = s.e0[2][c];
+    L.sE1[w][0][c * 64 + lane] = s.e0[0][c];
+    L.sE1[w][1][c * 64 + lane] = s.e0[2][c];
   }
   lds_barrier();
   float eup[2][CPL];
   if (w < NW - 1) {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      eup[0][c] = L.sE1[w + 1][0][zb + c];
-      eup[1][c] = L.sE1[w + 1][1][zb + c];
+      eup[0][c] = L.sE1[w + 1][0][c * 64 + lane];
+      eup[1][c] = L.sE1[w + 1][1][c * 64 + lane];
     }
   } else {
 #pragma unroll
@@ -331,8 +333,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
   }
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sH1[w][0][zb + c] = hn1[0][c];
-    L.sH1[w][1][zb + c] = hn1[2][c];
+    L.sH1[w][0][c * 64 + lane] = hn1[0][c];
+    L.sH1[w][1][c * 64 + lane] = hn1[2][c];
   }
   lds_barrier();
 
@@ -342,8 +344,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
     float hdn[2][CPL], hp[2][CPL], psE[2][CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      hdn[0][c] = L.sH1[w - 1][0][zb + c];
-      hdn[1][c] = L.sH1[w - 1][1][zb + c];
+      hdn[0][c] = L.sH1[w - 1][0][c * 64 + lane];
+      hdn[1][c] = L.sH1[w - 1][1][c * 64 + lane];
       hp[0][c] = s.h1m[1][c];
       hp[1][c] = s.h1m[2][c];
       psE[0][c] = cur.ps[2][c];
@@ -362,16 +364,16 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
   }
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sE2[PAR][w][0][zb + c] = en1[0][c];
-    L.sE2[PAR][w][1][zb + c] = en1[2][c];
+    L.sE2[PAR][w][0][c * 64 + lane] = en1[0][c];
+    L.sE2[PAR][w][1][c * 64 + lane] = en1[2][c];
   }
   float hn2[3][CPL], psH2[2][CPL];
   if (h2_rows && l2) {
     float e2up[2][CPL], psHm[2][CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      e2up[0][c] = L.sE2[PAR ^ 1][w + 1][0][zb + c];
-      e2up[1][c] = L.sE2[PAR ^ 1][w + 1][1][zb + c];
+      e2up[0][c] = L.sE2[PAR ^ 1][w + 1][0][c * 64 + lane];
+      e2up[1][c] = L.sE2[PAR ^ 1][w + 1][1][c * 64 + lane];
       psHm[0][c] = s.psm[0][c];
       psHm[1][c] = s.psm[1][c];
     }
@@ -385,8 +387,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
   }
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sH2[w][0][zb + c] = hn2[0][c];
-    L.sH2[w][1][zb + c] = hn2[2][c];
+    L.sH2[w][0][c * 64 + lane] = hn2[0][c];
+    L.sH2[w][1][c * 64 + lane] = hn2[2][c];
   }
   lds_barrier();
 
@@ -395,8 +397,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
     float hdn[2][CPL], psEm[2][CPL], en2[3][CPL], psE2[2][CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      hdn[0][c] = L.sH2[w - 1][0][zb + c];
-      hdn[1][c] = L.sH2[w - 1][1][zb + c];
+      hdn[0][c] = L.sH2[w - 1][0][c * 64 + lane];
+      hdn[1][c] = L.sH2[w - 1][1][c * 64 + lane];
       psEm[0][c] = s.psm[2][c];
       psEm[1][c] = s.psm[3][c];
     }
