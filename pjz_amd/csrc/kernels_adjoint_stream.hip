@@ -45,11 +45,14 @@ struct AdjStreamArgs {
   int nsteps;  // steps of the simulation (batched runs)
 };
 
-// Row exchange buffers (cell c of lane l at [c * 64 + l]: conflict-free banks).
+// Row exchange buffers (cell c of lane l at [c * 64 + l]: conflict-free
+// banks), double-buffered by plane parity: lK of plane x+1 is published from
+// the prefetch while plane x is computed, so one barrier per plane (between
+// the lH and the lE updates) orders everything.
 template <int CPL, int WY>
 struct AdjLds {
-  float sK[WY + 1][2][64 * CPL];  // lK x, z of plane x, per row
-  float sH[WY + 1][2][64 * CPL];  // new lH x, z of plane x, per row
+  float sK[2][WY + 1][2][64 * CPL];  // lK x, z of a plane, per row
+  float sH[2][WY + 1][2][64 * CPL];  // new lH x, z of a plane, per row
 };
 
 // Everything one wave reads for plane x of its row, fetched one plane ahead.
@@ -164,6 +167,12 @@ __device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLd
   }
   AdjPlane<CPL> cur, nxt;
   adj_fetch<CPL, WY, vec>(cur, a, xs, yc, w, zb, row_valid, owner && xs >= xa, pml, pidx);
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    sK[0][w][0][c * 64 + lane] = cb0[0][c] * le0[0][c];
+    sK[0][w][1][c * 64 + lane] = cb0[2][c] * le0[2][c];
+  }
+  lds_barrier();
 
   for (int x = xs; x < xb; ++x) {
     const bool ghost_plane = x < xa;
@@ -176,18 +185,18 @@ __device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLd
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int c = 0; c < CPL; ++c) K[k][c] = cb0[k][c] * le0[k][c];
+    const int par = (x - xs) & 1;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      sK[w][0][c * 64 + lane] = K[0][c];
-      sK[w][1][c * 64 + lane] = K[2][c];
+      sK[par ^ 1][w][0][c * 64 + lane] = cur.cb[0][c] * cur.le[0][c];
+      sK[par ^ 1][w][1][c * 64 + lane] = cur.cb[2][c] * cur.le[2][c];
     }
-    lds_barrier();
     float kyp[2][CPL];  // lK x, z of the row above
     if (w < WY) {
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        kyp[0][c] = sK[w + 1][0][c * 64 + lane];
-        kyp[1][c] = sK[w + 1][1][c * 64 + lane];
+        kyp[0][c] = sK[par][w + 1][0][c * 64 + lane];
+        kyp[1][c] = sK[par][w + 1][1][c * 64 + lane];
       }
     } else {
 #pragma unroll
@@ -237,8 +246,8 @@ __device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLd
     }
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      sH[w][0][c * 64 + lane] = h[0][c];
-      sH[w][1][c * 64 + lane] = h[2][c];
+      sH[par][w][0][c * 64 + lane] = h[0][c];
+      sH[par][w][1][c * 64 + lane] = h[2][c];
     }
     lds_barrier();
 
@@ -258,8 +267,8 @@ __device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLd
       const int pb = (x * Y + y) * a.NP;
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        const float hx_ym = sH[w - 1][0][c * 64 + lane];
-        const float hz_ym = sH[w - 1][1][c * 64 + lane];
+        const float hx_ym = sH[par][w - 1][0][c * 64 + lane];
+        const float hz_ym = sH[par][w - 1][1][c * 64 + lane];
         const float gx = dt * ((ikH[c] * h[1][c] - ikHm[c] * hyz[c]) - (h[2][c] - hz_ym)) -
                          (uy[c] - uyz[c]);
         const float gy = dt * (-(ikH[c] * h[0][c] - ikHm[c] * hxz[c]) + (h[2][c] - hzp[c])) -

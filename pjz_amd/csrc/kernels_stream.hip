@@ -118,9 +118,12 @@ k_stream(StreamArgs a) {
   const int xa = tx * a.LX;
   const int xb = min(xa + a.LX, X);
 
-  // Row exchange (cell c of lane l at [c * 64 + l]: conflict-free banks).
-  __shared__ float sE[WY + 1][2][64 * CPL];  // Ex, Ez of plane x, per row
-  __shared__ float sH[WY + 1][2][64 * CPL];  // new Hx, Hz of plane x, per row
+  // Row exchange (cell c of lane l at [c * 64 + l]: conflict-free banks),
+  // double-buffered by plane parity: the input Ex, Ez of plane x+1 are
+  // published from the prefetch while plane x is computed, so one barrier
+  // per plane (between the H and the E half steps) orders everything.
+  __shared__ float sE[2][WY + 1][2][64 * CPL];  // Ex, Ez of a plane, per row
+  __shared__ float sH[2][WY + 1][2][64 * CPL];  // new Hx, Hz of a plane, per row
 
   float ikE[CPL], bE[CPL], cE[CPL], ikH[CPL], bH[CPL], cH[CPL];
   bool pml[CPL];
@@ -152,6 +155,12 @@ k_stream(StreamArgs a) {
   Plane<CPL> cur, nxt;
   fetch_plane<CPL, WY, vec>(cur, a, xs, y, w, zb, row_valid, owner && xs >= xa, pml, pidx,
                             pstride);
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    sE[0][w][0][c * 64 + lane] = e0[0][c];
+    sE[0][w][1][c * 64 + lane] = e0[2][c];
+  }
+  lds_barrier();
 
   for (int x = xs; x < xb; ++x) {
     const bool ghost_plane = x < xa;
@@ -159,18 +168,18 @@ k_stream(StreamArgs a) {
     // Prefetch plane x+1 (zeros past the chunk).
     fetch_plane<CPL, WY, vec>(nxt, a, x + 1, y, w, zb, row_valid && x + 1 < xb,
                               owner && x + 1 < xb, pml, pidx, pstride);
+    const int par = (x - xs) & 1;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      sE[w][0][c * 64 + lane] = e0[0][c];
-      sE[w][1][c * 64 + lane] = e0[2][c];
+      sE[par ^ 1][w][0][c * 64 + lane] = cur.e[0][c];
+      sE[par ^ 1][w][1][c * 64 + lane] = cur.e[2][c];
     }
-    lds_barrier();
     float eup[2][CPL];
     if (w < WY) {
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        eup[0][c] = sE[w + 1][0][c * 64 + lane];
-        eup[1][c] = sE[w + 1][1][c * 64 + lane];
+        eup[0][c] = sE[par][w + 1][0][c * 64 + lane];
+        eup[1][c] = sE[par][w + 1][1][c * 64 + lane];
       }
     } else {
 #pragma unroll
@@ -214,8 +223,8 @@ k_stream(StreamArgs a) {
     }
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      sH[w][0][c * 64 + lane] = hn[0][c];
-      sH[w][1][c * 64 + lane] = hn[2][c];
+      sH[par][w][0][c * 64 + lane] = hn[0][c];
+      sH[par][w][1][c * 64 + lane] = hn[2][c];
     }
     lds_barrier();
 
@@ -227,8 +236,8 @@ k_stream(StreamArgs a) {
       float hdn[2][CPL];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        hdn[0][c] = sH[w - 1][0][c * 64 + lane];
-        hdn[1][c] = sH[w - 1][1][c * 64 + lane];
+        hdn[0][c] = sH[par][w - 1][0][c * 64 + lane];
+        hdn[1][c] = sH[par][w - 1][1][c * 64 + lane];
       }
       float en[3][CPL];
 #pragma unroll

@@ -275,14 +275,16 @@ __device__ __forceinline__ void store_rare(const Stream2Args& a, bool snap, bool
 // 64 lanes of an access hit consecutive banks.
 template <int CPL, int WY>
 struct Lds2 {
-  float sE1[WY + 3][2][64 * CPL];
+  float sE1[2][WY + 3][2][64 * CPL];
   float sH1[WY + 3][2][64 * CPL];
   float sE2[2][WY + 3][2][64 * CPL];
   float sH2[WY + 3][2][64 * CPL];
 };
 
 // One plane of the sweep: level 1 at x, level 2 at x-1.  PAR selects the
-// sE2 slot written this plane (the other slot holds plane x-1).
+// sE2 slot written this plane (the other slot holds plane x-1) and the sE1
+// slot read this plane (input Ex, Ez of plane x, published one plane early
+// from the prefetched registers, so level-1 H needs no barrier of its own).
 template <int CPL, int WY, bool vec>
 __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, Carry2<CPL>& s,
                                        const Plane2<CPL>& cur, Plane2<CPL>& nxt,
@@ -300,19 +302,20 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
   if (x + 1 <= xb)
     fetch2<CPL, vec>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, e_rows, zc);
 
-  // ---- A: level-1 H at plane x
+  // ---- A: level-1 H at plane x (input E of the row above from sE1[PAR];
+  // this row's input E of plane x+1 goes to sE1[PAR ^ 1], read next plane
+  // after this plane's barriers)
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sE1[w][0][c * 64 + lane] = s.e0[0][c];
-    L.sE1[w][1][c * 64 + lane] = s.e0[2][c];
+    L.sE1[PAR ^ 1][w][0][c * 64 + lane] = cur.e[0][c];
+    L.sE1[PAR ^ 1][w][1][c * 64 + lane] = cur.e[2][c];
   }
-  lds_barrier();
   float eup[2][CPL];
   if (w < NW - 1) {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      eup[0][c] = L.sE1[w + 1][0][c * 64 + lane];
-      eup[1][c] = L.sE1[w + 1][1][c * 64 + lane];
+      eup[0][c] = L.sE1[PAR][w + 1][0][c * 64 + lane];
+      eup[1][c] = L.sE1[PAR][w + 1][1][c * 64 + lane];
     }
   } else {
 #pragma unroll
@@ -496,6 +499,13 @@ __device__ __forceinline__ void stream2_tile(const Stream2Args& a, int tile, Lds
   }
   Plane2<CPL> cur, nxt;
   fetch2<CPL, vec>(cur, a, xs, yc, w == NW - 1, zb, row_valid, w >= 1, zc);
+  // Input Ex, Ez of the first plane for the row below's level-1 H.
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    L.sE1[0][w][0][c * 64 + lane] = s.e0[0][c];
+    L.sE1[0][w][1][c * 64 + lane] = s.e0[2][c];
+  }
+  lds_barrier();
   int par = 0;
   for (int x = xs; x <= xb; ++x) {
     plane2<CPL, WY, vec>(a, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid, par);
