@@ -281,16 +281,36 @@ struct Lds2 {
   float sH2[WY + 3][2][64 * CPL];
 };
 
+// Level-2 outputs (E, H at Eout + hoff, new psi) written through buffer
+// resources: a row or lane with nothing to store uses an out-of-range offset
+// (kOff) and its store is dropped, so every plane issues the same stores and
+// the wait for the next plane's prefetched loads does not also wait for them.
+struct Out2 {
+  rsrc_t eh, psi;
+  unsigned N4, H4, P4;
+};
+
+__device__ inline Out2 make_out2(const Stream2Args& a) {
+  Out2 o;
+  o.N4 = 4u * (unsigned)a.N;
+  o.H4 = 4u * (unsigned)a.hoff;
+  o.P4 = 4u * (unsigned)a.P;
+  o.eh = make_rsrc(a.Eout, o.H4 + 3 * o.N4);
+  o.psi = make_rsrc(a.psiOut, 4 * o.P4);
+  return o;
+}
+
 // One plane of the sweep: level 1 at x, level 2 at x-1.  PAR selects the
 // sE2 slot written this plane (the other slot holds plane x-1) and the sE1
 // slot read this plane (input Ex, Ez of plane x, published one plane early
 // from the prefetched registers, so level-1 H needs no barrier of its own).
 template <int CPL, int WY, bool vec>
-__device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, Carry2<CPL>& s,
+__device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2<CPL, WY>& L,
+                                       Carry2<CPL>& s,
                                        const Plane2<CPL>& cur, Plane2<CPL>& nxt,
                                        const ZCoef<CPL>& zc, int x, int xa, int xb, int w,
                                        int y, int yc, int lane, int zb, bool row_valid,
-                                       int PAR) {
+                                       bool lane_ok, int PAR) {
   constexpr int NW = WY + 3;
   const bool e_rows = w >= 1;
   const bool h2_rows = w >= 1 && w <= WY + 1;
@@ -396,8 +416,10 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
   lds_barrier();
 
   // ---- C: level-2 E at plane x-1 (the result on owned rows)
+  const bool st = own && l2 && row_valid && x - 1 >= xa;
+  float en2[3][CPL], psE2[2][CPL];
   if (own && l2 && row_valid) {
-    float hdn[2][CPL], psEm[2][CPL], en2[3][CPL], psE2[2][CPL];
+    float hdn[2][CPL], psEm[2][CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
       hdn[0][c] = L.sH2[w - 1][0][c * 64 + lane];
@@ -407,27 +429,31 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, Lds2<CPL, WY>& L, C
     }
     e_row<CPL>(en2, psE2, s.e1m, hn2, hdn, s.h2p, s.cbm, s.sigm, psEm, zc, a, a.n + 1, x - 1, y,
                zb, lane, true);
-    if (x - 1 >= xa) {
-      const int o = ((x - 1) * a.Y + y) * Z;
+    if (st && (a.flags & (SNAP2 | CAPTURE)))
+      store_rare<CPL, vec>(a, a.flags & SNAP2, a.flags & CAPTURE, a.n + 1, x - 1, y, zb, en2,
+                           s.e1m);
+  } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        store_col<CPL, vec>(a.Eout + (k * N + o), en2[k], zb, Z);
-        store_col<CPL, vec>(a.Eout + (a.hoff + k * N + o), hn2[k], zb, Z);
-      }
-      const int pb = ((x - 1) * a.Y + y) * a.NP;
+    for (int c = 0; c < CPL; ++c) {
+      en2[0][c] = en2[1][c] = en2[2][c] = 0.0f;
+      psE2[0][c] = psE2[1][c] = 0.0f;
+    }
+  }
+  {
+    const unsigned ob = st && lane_ok ? (unsigned)((((x - 1) * a.Y + y) * Z + zb) * 4) : kOff;
 #pragma unroll
-      for (int c = 0; c < CPL; ++c) {
-        if (zc.pml[c]) {
-          const int pi = pb + zc.pidx[c];
-          a.psiOut[pi] = psH2[0][c];
-          a.psiOut[a.P + pi] = psH2[1][c];
-          a.psiOut[2 * a.P + pi] = psE2[0][c];
-          a.psiOut[3 * a.P + pi] = psE2[1][c];
-        }
-      }
-      if (a.flags & (SNAP2 | CAPTURE))
-        store_rare<CPL, vec>(a, a.flags & SNAP2, a.flags & CAPTURE, a.n + 1, x - 1, y, zb, en2,
-                             s.e1m);
+    for (int k = 0; k < 3; ++k) {
+      bstore<CPL>(en2[k], O.eh, ob, k * O.N4);
+      bstore<CPL>(hn2[k], O.eh, ob, O.H4 + k * O.N4);
+    }
+    const int pb = ((x - 1) * a.Y + y) * a.NP;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      const unsigned vp = st && zc.pml[c] ? (unsigned)((pb + zc.pidx[c]) * 4) : kOff;
+      bstore1(psH2[0][c], O.psi, vp, 0);
+      bstore1(psH2[1][c], O.psi, vp, O.P4);
+      bstore1(psE2[0][c], O.psi, vp, 2 * O.P4);
+      bstore1(psE2[1][c], O.psi, vp, 3 * O.P4);
     }
   }
 
@@ -506,9 +532,12 @@ __device__ __forceinline__ void stream2_tile(const Stream2Args& a, int tile, Lds
     L.sE1[0][w][1][c * 64 + lane] = s.e0[2][c];
   }
   lds_barrier();
+  const Out2 O = make_out2(a);
+  const bool lane_ok = zb < Z;
   int par = 0;
   for (int x = xs; x <= xb; ++x) {
-    plane2<CPL, WY, vec>(a, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid, par);
+    plane2<CPL, WY, vec>(a, O, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid,
+                         lane_ok, par);
     cur = nxt;
     par ^= 1;
   }
@@ -563,8 +592,13 @@ static void launch2(Stream2Args& a, hipStream_t s) {
 }
 
 // Whether the two-step kernel handles this problem (vector loads needed).
+// Buffer resources address at most 2^31 bytes: E and H of one level (24 B a
+// cell plus alignment) must fit, i.e. up to about 89M cells.
+static bool fits_rsrc(const Geom& g) { return g.N * 24 + 1024 <= (1LL << 31); }
+
 bool stream2_supported(const Geom& g) {
-  return g.Z <= 256 && ((g.Z <= 64) || (g.Z <= 128 && g.Z % 2 == 0) || (g.Z % 4 == 0));
+  return fits_rsrc(g) && g.Z <= 256 &&
+         ((g.Z <= 64) || (g.Z <= 128 && g.Z % 2 == 0) || (g.Z % 4 == 0));
 }
 
 // Runs steps [r.begin, r.end) two at a time; an odd last step uses the
@@ -693,7 +727,9 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
 // ---------------------------------------------------------------- batches
 // Simulations of a batch are grouped by cells per lane (zz <= 64: 1, even
 // zz <= 128: 2); each group runs as one persistent launch per step pair.
-bool batch2_supported(const Geom& g) { return g.Z <= 64 || (g.Z <= 128 && g.Z % 2 == 0); }
+bool batch2_supported(const Geom& g) {
+  return fits_rsrc(g) && (g.Z <= 64 || (g.Z <= 128 && g.Z % 2 == 0));
+}
 
 static constexpr int kBatchWY = 8;
 static constexpr int kBatchLX = 128;
