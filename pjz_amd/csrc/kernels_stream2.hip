@@ -58,6 +58,7 @@ struct Stream2Args {
   int saxis, spos, squad;
   float dt;
   int n, flags;  // first step of the pair; Stream2Flags of this launch
+  int pair;      // CPL = 2 with even PML widths: psi entries in aligned pairs
   int LX, nty, ntiles;
   int nsteps;  // steps of the simulation (batched runs)
 };
@@ -222,18 +223,32 @@ __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y
     load_col<CPL, vec>(P.up[1], a.Ein + (2 * N + o + Z), zb, Z, v);
   }
   const int pb = (x * a.Y + yc) * a.NP;
+  if (CPL == 2 && a.pair) {
+    // Even PML widths and Z: a lane's two cells are both PML or both not,
+    // with consecutive, 8-byte aligned psi entries: one 64-bit load each.
+    const bool ph = zc.pml[0] && active;
+    const int pi = pb + zc.pidx[0];
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const bool ph = zc.pml[c] && active;
-    const int pi = pb + zc.pidx[c];
-    P.ps[0][c] = ph ? a.psiIn[pi] : 0.0f;
-    P.ps[1][c] = ph ? a.psiIn[a.P + pi] : 0.0f;
-    P.ps[2][c] = ph && e_rows ? a.psiIn[2 * a.P + pi] : 0.0f;
-    P.ps[3][c] = ph && e_rows ? a.psiIn[3 * a.P + pi] : 0.0f;
+    for (int k = 0; k < 4; ++k) {
+      float2 t = make_float2(0.0f, 0.0f);
+      if (ph && (k < 2 || e_rows)) t = *reinterpret_cast<const float2*>(a.psiIn + (k * a.P + pi));
+      P.ps[k][0] = t.x;
+      P.ps[k][CPL - 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      const bool ph = zc.pml[c] && active;
+      const int pi = pb + zc.pidx[c];
+      P.ps[0][c] = ph ? a.psiIn[pi] : 0.0f;
+      P.ps[1][c] = ph ? a.psiIn[a.P + pi] : 0.0f;
+      P.ps[2][c] = ph && e_rows ? a.psiIn[2 * a.P + pi] : 0.0f;
+      P.ps[3][c] = ph && e_rows ? a.psiIn[3 * a.P + pi] : 0.0f;
+    }
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k)
-    P.sig[k] = active && e_rows ? a.mask[(k * a.X + x) * a.Y + yc] : 0.0f;
+    P.sig[k] = active && e_rows ? uniform_load(a.mask, (k * a.X + x) * a.Y + yc) : 0.0f;
 }
 
 // Writes the snapshot of E after `step` (en) and/or the capture of its change
@@ -447,13 +462,21 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
       bstore<CPL>(hn2[k], O.eh, ob, O.H4 + k * O.N4);
     }
     const int pb = ((x - 1) * a.Y + y) * a.NP;
+    if (CPL == 2 && a.pair) {
+      const unsigned vp = st && zc.pml[0] ? (unsigned)((pb + zc.pidx[0]) * 4) : kOff;
+      bstore<CPL>(psH2[0], O.psi, vp, 0);
+      bstore<CPL>(psH2[1], O.psi, vp, O.P4);
+      bstore<CPL>(psE2[0], O.psi, vp, 2 * O.P4);
+      bstore<CPL>(psE2[1], O.psi, vp, 3 * O.P4);
+    } else {
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      const unsigned vp = st && zc.pml[c] ? (unsigned)((pb + zc.pidx[c]) * 4) : kOff;
-      bstore1(psH2[0][c], O.psi, vp, 0);
-      bstore1(psH2[1][c], O.psi, vp, O.P4);
-      bstore1(psE2[0][c], O.psi, vp, 2 * O.P4);
-      bstore1(psE2[1][c], O.psi, vp, 3 * O.P4);
+      for (int c = 0; c < CPL; ++c) {
+        const unsigned vp = st && zc.pml[c] ? (unsigned)((pb + zc.pidx[c]) * 4) : kOff;
+        bstore1(psH2[0][c], O.psi, vp, 0);
+        bstore1(psH2[1][c], O.psi, vp, O.P4);
+        bstore1(psE2[0][c], O.psi, vp, 2 * O.P4);
+        bstore1(psE2[1][c], O.psi, vp, 3 * O.P4);
+      }
     }
   }
 
@@ -665,6 +688,7 @@ static Stream2Args make_args2(const Geom& g, const pjz_fdtd_problem& p, const fl
   a.dt = p.dt;
   a.n = 0;
   a.flags = 0;
+  a.pair = g.Z > 64 && g.Z <= 128 && g.Z % 2 == 0 && g.lo % 2 == 0 && g.hi % 2 == 0;
   a.LX = LX;
   a.nty = (g.Y + WY - 1) / WY;
   a.ntiles = a.nty * ((g.X + LX - 1) / LX);

@@ -112,6 +112,12 @@ __device__ inline void bstore(const float (&v)[CPL], rsrc_t r, unsigned voff, un
   }
 }
 
+// Read-only data at a wave-uniform index, loaded through the scalar cache
+// (the constant address space lets the compiler use a scalar load), which
+// keeps it off the vector memory pipeline.
+typedef const __attribute__((address_space(4))) float* const_fptr;
+__device__ inline float uniform_load(const float* p, int i) { return ((const_fptr)p)[i]; }
+
 // Cell z+1 for each of the lane's cells (0 at and beyond Z).
 template <int CPL>
 __device__ inline void z_next(float (&o)[CPL], const float (&v)[CPL], int zb, int Z) {
