@@ -55,6 +55,9 @@ extern "C" {
 #define PJZ_FDTD_KERNEL_REFERENCE 1 /* two launches per step, one thread/cell */
 #define PJZ_FDTD_KERNEL_STREAM 2    /* one fused launch per step, x-sweep    */
 #define PJZ_FDTD_KERNEL_STREAM2 3   /* two steps per launch (temporal block) */
+#define PJZ_FDTD_KERNEL_AUTOTUNE 4  /* first call per shape times the kernels and */
+                                    /* tilings above (synchronising the stream), */
+                                    /* later calls reuse the fastest             */
 
 /* Problem description: the static arguments of fdtdz(...). */
 typedef struct pjz_fdtd_problem {

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libpjzfdtd.so")
 ABI_VERSION = 1
 
 OK, EINVAL, EHIP, ESPACE = 0, 1, 2, 3
-KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_STREAM, KERNEL_STREAM2 = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_STREAM, KERNEL_STREAM2, KERNEL_AUTOTUNE = 0, 1, 2, 3, 4
 
 # Every entry point include/pjz_fdtd.h declares.
 EXPORTS = (

@@ -5,6 +5,10 @@
 #include <stdarg.h>
 #include <limits.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "fdtd_common.h"
@@ -74,7 +78,7 @@ int check(const pjz_fdtd_problem* p) {
       return fail(PJZ_FDTD_EINVAL, "last output step %d needs waveform rows > %d (tt=%d)", last,
                   last, p->tt);
   }
-  if (p->kernel < PJZ_FDTD_KERNEL_AUTO || p->kernel > PJZ_FDTD_KERNEL_STREAM2)
+  if (p->kernel < PJZ_FDTD_KERNEL_AUTO || p->kernel > PJZ_FDTD_KERNEL_AUTOTUNE)
     return fail(PJZ_FDTD_EINVAL, "unknown kernel id %d", p->kernel);
   if (p->kernel == PJZ_FDTD_KERNEL_STREAM && p->zz > 256)
     return fail(PJZ_FDTD_EINVAL, "the stream kernel needs zz <= 256, got %d", p->zz);
@@ -97,12 +101,128 @@ pjz::Geom geom_of(const pjz_fdtd_problem& p) {
   return g;
 }
 
-// Steps [b, end) with the kernel the problem selects (AUTO: stream if zz <= 256).
+// ---------------------------------------------------------------- autotuning
+// Kernel and tiling measured fastest for a problem shape on a device; the
+// cache is process-wide and guarded by a mutex.
+struct Tuned {
+  int kernel, tile_x, tile_y;
+};
+using TuneKey = std::tuple<int, int, int, int, int, int>;  // device, xx, yy, zz, lo, hi
+std::mutex g_tune_mu;
+std::map<TuneKey, Tuned> g_tuned;
+
+TuneKey tune_key(const pjz_fdtd_problem& p) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return TuneKey{dev, p.xx, p.yy, p.zz, p.pml_lo, p.pml_hi};
+}
+
+bool tuned_choice(const pjz_fdtd_problem& p, Tuned* t) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tuned.find(tune_key(p));
+  if (it == g_tuned.end()) return false;
+  *t = it->second;
+  return true;
+}
+
+hipError_t run_steps(const pjz::Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                     const pjz::Src& src, const pjz::OutWin& win, char* ws,
+                     const pjz::Layout& L, int b, int end, int* cur, float* capture,
+                     hipStream_t s);
+
+hipError_t clear_state(const pjz::Geom& g, const pjz::Layout& L, char* ws, hipStream_t s) {
+  hipError_t e = pjz::launch_zero((float*)(ws + L.E0), (L.E1 - L.E0) / sizeof(float), s);
+  if (e == hipSuccess && g.NP > 0)
+    e = pjz::launch_zero((float*)(ws + L.psi), (L.zt - L.psi) / sizeof(float), s);
+  return e;
+}
+
+// Times the candidate kernels and tilings on a few steps of this problem
+// (from a cleared state, no snapshots, on the caller's workspace and
+// stream), caches the fastest for the shape and leaves the state cleared.
+// Synchronises the stream: only the first call for a shape pays for it.
+hipError_t autotune(const pjz::Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                    const pjz::Src& src, char* ws, const pjz::Layout& L, hipStream_t s) {
+  std::vector<Tuned> cands;
+  const bool cpl2 = g.Z > 64 && g.Z <= 128 && g.Z % 2 == 0;
+  if (pjz::stream2_supported(g)) {
+    cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 0, 0});
+    cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 64, 0});
+    if (cpl2) cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 0, 4});
+  }
+  if (g.Z <= 256) {
+    cands.push_back({PJZ_FDTD_KERNEL_STREAM, 0, 0});
+    if (cpl2) cands.push_back({PJZ_FDTD_KERNEL_STREAM, 0, 4});
+  }
+  cands.push_back({PJZ_FDTD_KERNEL_REFERENCE, 0, 0});
+  // Up to 8 steps, within the waveform's rows (the run's own step count).
+  const int nsteps = std::min(8, pjz::steps_of(p)) & ~1;
+  if (nsteps < 2) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned[tune_key(p)] = cands[0];
+    return hipSuccess;
+  }
+  pjz::OutWin nowin{p.x0, p.y0, p.z0, p.ex, p.ey, p.ez, nullptr};
+  hipEvent_t t0, t1;
+  hipError_t e = hipEventCreate(&t0);
+  if (e != hipSuccess) return e;
+  e = hipEventCreate(&t1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(t0);
+    return e;
+  }
+  Tuned best = cands[0];
+  float best_ms = 1e30f;
+  for (size_t i = 0; i < cands.size() && e == hipSuccess; ++i) {
+    pjz_fdtd_problem q = p;
+    q.kernel = cands[i].kernel;
+    q.tile_x = cands[i].tile_x;
+    q.tile_y = cands[i].tile_y;
+    // One untimed pass loads the code, then the timed one.
+    for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+      int cur = 0;
+      e = clear_state(g, L, ws, s);
+      if (e == hipSuccess && rep == 1) e = hipEventRecord(t0, s);
+      if (e == hipSuccess) e = run_steps(g, q, mask, src, nowin, ws, L, 0, nsteps, &cur, nullptr, s);
+      if (e == hipSuccess && rep == 1) e = hipEventRecord(t1, s);
+    }
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventSynchronize(t1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+    if (e == hipSuccess && ms < best_ms) {
+      best_ms = ms;
+      best = cands[i];
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (e == hipSuccess) e = clear_state(g, L, ws, s);
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned[tune_key(p)] = best;
+  }
+  return e;
+}
+
+// Steps [b, end) with the kernel the problem selects (AUTO: the two-step
+// kernel where supported, else the one-step kernel if zz <= 256; AUTOTUNE:
+// the cached choice for the shape, else AUTO).
 hipError_t run_steps(const pjz::Geom& g, const pjz_fdtd_problem& p, const float* mask,
                      const pjz::Src& src, const pjz::OutWin& win, char* ws,
                      const pjz::Layout& L, int b, int end, int* cur, float* capture,
                      hipStream_t s) {
   int kernel = p.kernel;
+  Tuned t;
+  if (kernel == PJZ_FDTD_KERNEL_AUTOTUNE) {
+    if (tuned_choice(p, &t)) {
+      pjz_fdtd_problem q = p;
+      q.kernel = t.kernel;
+      q.tile_x = t.tile_x;
+      q.tile_y = t.tile_y;
+      return run_steps(g, q, mask, src, win, ws, L, b, end, cur, capture, s);
+    }
+    kernel = PJZ_FDTD_KERNEL_AUTO;
+  }
   if (kernel == PJZ_FDTD_KERNEL_AUTO)
     kernel = pjz::stream2_supported(g) ? PJZ_FDTD_KERNEL_STREAM2
              : g.Z <= 256             ? PJZ_FDTD_KERNEL_STREAM
@@ -212,6 +332,11 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
     e = pjz::launch_zero((float*)(ws + L.psi), (L.zt - L.psi) / sizeof(float), s);
   if (e != hipSuccess) return hip_fail(e, "clear state");
   pjz::Src src{p->src_axis, p->src_pos, p->src_quad, source_field, source_waveform};
+  Tuned tuned;
+  if (p->kernel == PJZ_FDTD_KERNEL_AUTOTUNE && !tuned_choice(*p, &tuned)) {
+    e = autotune(g, *p, absorption_mask, src, ws, L, s);
+    if (e != hipSuccess) return hip_fail(e, "autotune");
+  }
   pjz::OutWin win{p->x0, p->y0, p->z0, p->ex, p->ey, p->ez, out};
   const int nsteps = pjz::steps_of(*p);
   int cur = 0;

@@ -701,7 +701,9 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
                        const StepRange& r, hipStream_t s) {
   if (!stream2_supported(g)) return hipErrorInvalidValue;
   const int cpl = g.Z <= 64 ? 1 : (g.Z <= 128 ? 2 : 4);
-  const int WY = cpl == 4 ? 4 : (p.tile_y > 0 ? p.tile_y : 8);
+  // Row tiles with a compiled kernel: 8 (4 at CPL = 4); CPL = 2 also 4, 12, 13.
+  int WY = cpl == 4 ? 4 : 8;
+  if (cpl == 2 && (p.tile_y == 4 || p.tile_y == 12 || p.tile_y == 13)) WY = p.tile_y;
   int LX = p.tile_x;
   if (LX <= 0) {
     // x chunks sized so the grid has about one workgroup per CU.

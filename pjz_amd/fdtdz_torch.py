@@ -20,14 +20,17 @@ import torch
 from . import _lib
 
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "reference": _lib.KERNEL_REFERENCE,
-            "stream": _lib.KERNEL_STREAM, "stream2": _lib.KERNEL_STREAM2}
+            "stream": _lib.KERNEL_STREAM, "stream2": _lib.KERNEL_STREAM2,
+            "autotune": _lib.KERNEL_AUTOTUNE}
 _LAUNCH_KEYS = {"kernel", "tile_x", "tile_y", "waves_per_simd", "ckpt_every"}
 
 
 def _launch_options(launch_params):
   """Parses ``launch_params``: None, a mapping, or a tuple of (key, value).
 
-  Keys: ``kernel`` ("auto" | "reference" | "stream" | "stream2"), ``tile_x``, ``tile_y``,
+  Keys: ``kernel`` ("auto" | "reference" | "stream" | "stream2" | "autotune": the
+  first call per shape times the kernels and tilings, later calls reuse the
+  fastest), ``tile_x``, ``tile_y``,
   ``waves_per_simd`` (stream kernel tiling), ``ckpt_every`` (steps between
   checkpoints kept for the gradient; default about sqrt(2 * steps)).
   """

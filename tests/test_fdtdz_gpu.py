@@ -162,3 +162,22 @@ def test_stream_launch_variants_match_oracle(tile_x, kernel, tile_y):
   kw["launch_params"] = dict(kernel=kernel, tile_x=tile_x, tile_y=tile_y)
   got = F.fdtdz(**kw).cpu().numpy()
   assert rel_l2(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("Z", [32, 64, 96])
+def test_autotune_matches_oracle_and_reuses_choice(Z):
+  """The autotuned call (first call times the candidates on the caller's
+  workspace, then clears it) gives oracle-exact fields, and the second call
+  (cached choice, no tuning) gives the same fields bit for bit."""
+  kw = random_problem(1, Z=Z, seed=20 + Z, widths=(4, 6))
+  ref = C.fdtdz(**kw)
+  first = run_gpu(kw, "autotune")
+  second = run_gpu(kw, "autotune")
+  assert rel_l2(first, ref) <= TOL
+  np.testing.assert_array_equal(first, second)
+
+
+def test_autotune_with_a_single_step():
+  kw = random_problem(0, seed=31)
+  kw.update(output_steps=(0, 1, 1))
+  assert rel_l2(run_gpu(kw, "autotune"), C.fdtdz(**kw)) <= TOL
