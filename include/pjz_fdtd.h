@@ -77,7 +77,9 @@ typedef struct pjz_fdtd_problem {
   int32_t waves_per_simd;/* occupancy hint for the stream kernel, 0 = auto      */
   int32_t ckpt_every;    /* > 0: forward saves a checkpoint every ckpt_every     */
                          /* steps so pjz_fdtd_backward can run on the workspace  */
-  int32_t reserved[2];
+  int32_t graph_steps;   /* batches: steps per captured, replayed hipGraph    */
+                         /* chunk (even, <= 128); 0 = one launch per pair   */
+  int32_t reserved;
 } pjz_fdtd_problem;
 
 /* ABI version of the loaded library (PJZ_FDTD_ABI_VERSION). */

@@ -493,6 +493,8 @@ int pjz_fdtd_batch_forward(const pjz_fdtd_problem* probs, int n, const float* co
         e = copy_state(items[i].g, bp.Ls[i], items[i].ws, (b / 2) & 1, k, true, s);
     if (e == hipSuccess) e = pjz::batch2_pairs(plan, b / 2, (b + bp.S) / 2, s);
   }
+  const hipError_t r = pjz::batch2_release(plan, s);
+  if (e == hipSuccess) e = r;
   if (e != hipSuccess) return hip_fail(e, "batch launch");
   return PJZ_FDTD_OK;
 }
@@ -563,6 +565,8 @@ int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n, const float* c
             ? pjz::launch_zero(grad_eps[i], 3 * (size_t)probs[i].ex * probs[i].ey * probs[i].ez, s)
             : pjz::launch_grad_eps(items[i].g, probs[i], epsilon[i], absorption_mask[i],
                                    items[i].ws, bp.Ls[i], grad_eps[i], s);
+  const hipError_t r = pjz::batch2_release(plan, s);
+  if (e == hipSuccess) e = r;
   if (e != hipSuccess) return hip_fail(e, "batch backward launch");
   return PJZ_FDTD_OK;
 }

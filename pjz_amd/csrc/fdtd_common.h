@@ -170,6 +170,9 @@ struct BatchItem {
 // set m & 1 (a simulation skips pairs past its last step).
 struct Batch2Plan {
   int n, npairs, cus;
+  int chunk;               // step pairs per replayed graph chunk (0: direct launches)
+  hipGraph_t graph;        // captured chunk, built on first use
+  hipGraphExec_t exec;
   int count[2], base[2], total[2];  // per cells-per-lane class: sims, first sim, tiles
   char* meta;
   size_t args_bytes, rare_bytes, ts_bytes;
@@ -178,7 +181,9 @@ bool batch2_supported(const Geom& g);
 size_t batch2_meta_bytes(const BatchItem* items, int n);
 hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan& plan,
                           hipStream_t s);
-hipError_t batch2_pairs(const Batch2Plan& plan, int m0, int m1, hipStream_t s);
+hipError_t batch2_pairs(Batch2Plan& plan, int m0, int m1, hipStream_t s);
+// Frees the plan's graph (synchronising the stream if one was built).
+hipError_t batch2_release(Batch2Plan& plan, hipStream_t s);
 hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s);
 hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s);
 

@@ -576,6 +576,10 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2(Stream2Args a) {
   stream2_tile<CPL, WY, vec>(a, tile, L);
 }
 
+__device__ inline int uniform_load_int(const int* p) {
+  return *(const __attribute__((address_space(4))) int*)p;
+}
+
 // Batched two-step kernel: persistent workgroups take (simulation, tile) work
 // items from a device counter until the launch's items are exhausted; the
 // simulation of item t is the last s with tile_start[s] <= t (tile counts are
@@ -585,9 +589,12 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2(Stream2Args a) {
 template <int CPL, int WY, bool vec>
 __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_batch(
     const Stream2Args* __restrict__ args, const int* __restrict__ tile_start, int nsims,
-    int total, int* __restrict__ counter, int step) {
+    int total, int* __restrict__ counter, const int* __restrict__ step_base, int step_off) {
   __shared__ Lds2<CPL, WY> L;
   __shared__ int s_item;
+  // First step of the pair: step_off, plus a device-held base when the launch
+  // is a node of a replayed graph chunk.
+  const int step = (step_base ? uniform_load_int(step_base) : 0) + step_off;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
@@ -759,13 +766,15 @@ bool batch2_supported(const Geom& g) {
 
 static constexpr int kBatchWY = 8;
 static constexpr int kBatchLX = 128;
+static constexpr int kMaxChunkPairs = 64;  // step pairs per captured graph chunk
 
 size_t batch2_meta_bytes(const BatchItem* items, int n) {
   int npairs = 0;
   for (int i = 0; i < n; ++i) npairs = std::max(npairs, (steps_of(*items[i].p) + 1) / 2);
   return align256(sizeof(Stream2Args) * 2 * (size_t)n) + align256(sizeof(Stream2Rare) * (size_t)n) +
          align256(sizeof(int) * (size_t)(n + 2) * 2) +
-         align256(sizeof(int) * 2 * (size_t)std::max(npairs, 1));
+         align256(sizeof(int) * 2 * (size_t)std::max(npairs, 1)) + align256(sizeof(int)) +
+         align256(sizeof(int) * 2 * (size_t)kMaxChunkPairs);
 }
 
 hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan& plan,
@@ -775,6 +784,9 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
   plan.npairs = 0;
   plan.meta = meta;
   plan.cus = device_cus();
+  plan.graph = nullptr;
+  plan.exec = nullptr;
+  plan.chunk = std::min(std::max(items[0].p->graph_steps / 2, 0), kMaxChunkPairs);
   for (int i = 0; i < n; ++i) {
     if (!batch2_supported(items[i].g)) return hipErrorInvalidValue;
     cls[items[i].g.Z <= 64 ? 0 : 1].push_back(i);
@@ -782,7 +794,8 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
   }
   // Host image of the argument views, launch data and tile prefix sums,
   // copied once.  Layout: args (2 per sim, class order) | rare (class order)
-  // | tile starts (n_c + 1 per class) | queue counters (2 per pair).
+  // | tile starts (n_c + 1 per class) | queue counters (2 per pair) | graph
+  // chunk step base | graph chunk queue counters.
   plan.args_bytes = align256(sizeof(Stream2Args) * 2 * (size_t)n);
   plan.rare_bytes = align256(sizeof(Stream2Rare) * (size_t)n);
   plan.ts_bytes = align256(sizeof(int) * (size_t)(n + 2) * 2);
@@ -818,34 +831,101 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
   return hipStreamSynchronize(s);
 }
 
-hipError_t batch2_pairs(const Batch2Plan& plan, int m0, int m1, hipStream_t s) {
-  m1 = std::min(m1, plan.npairs);
-  if (m0 >= m1) return hipSuccess;
-  char* meta = plan.meta;
-  int* counters = (int*)(meta + plan.args_bytes + plan.rare_bytes + plan.ts_bytes);
-  // Queue counters of these pairs start at zero (a fill kernel, so the
-  // sequence can be captured in a graph).
-  hipError_t e = launch_zero((float*)(counters + 2 * m0), 2 * (size_t)(m1 - m0), s);
+// The launches of one step pair (one per class), queue counters ctr[0..1]
+// (zero on entry), first step *step_base + step_off (step_base may be null).
+static void launch_pair(const Batch2Plan& plan, hipStream_t s, int* ctr, const int* step_base,
+                        int step_off) {
+  const Stream2Args* dargs = (const Stream2Args*)plan.meta;
+  const int* dts = (const int*)(plan.meta + plan.args_bytes + plan.rare_bytes);
+  for (int c = 0; c < 2; ++c) {
+    const int nc = plan.count[c];
+    if (nc == 0) continue;
+    const int grid = std::min(plan.total[c], plan.cus);
+    const Stream2Args* a = dargs + 2 * plan.base[c];
+    const int* ts = dts + plan.base[c] + c;
+    if (c == 0)
+      hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY, true>), dim3(grid),
+                         dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,
+                         step_base, step_off);
+    else
+      hipLaunchKernelGGL((k_stream2_batch<2, kBatchWY, true>), dim3(grid),
+                         dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,
+                         step_base, step_off);
+  }
+}
+
+__global__ void k_set_int(int* p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+__global__ void k_add_int(int* p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+
+static int* chunk_base(const Batch2Plan& plan) {
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.rare_bytes + plan.ts_bytes);
+  return (int*)((char*)counters + align256(sizeof(int) * 2 * (size_t)std::max(plan.npairs, 1)));
+}
+
+// Captures one chunk of plan.chunk step pairs as a graph: clear the chunk's
+// queue counters (a fill kernel: no memset nodes), the pair launches reading
+// their first step from the device-held base, then advance the base, so the
+// same executable graph replays for every chunk.
+static hipError_t build_chunk_graph(Batch2Plan& plan) {
+  int* base = chunk_base(plan);
+  int* ctr = base + 256 / sizeof(int);
+  hipStream_t cs;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e != hipSuccess) return e;
-  const Stream2Args* dargs = (const Stream2Args*)meta;
-  const int* dts = (const int*)(meta + plan.args_bytes + plan.rare_bytes);
-  for (int m = m0; m < m1; ++m) {
-    for (int c = 0; c < 2; ++c) {
-      const int nc = plan.count[c];
-      if (nc == 0) continue;
-      const int grid = std::min(plan.total[c], plan.cus);
-      const Stream2Args* a = dargs + 2 * plan.base[c];
-      const int* ts = dts + plan.base[c] + c;
-      int* ctr = counters + 2 * m + c;
-      if (c == 0)
-        hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY, true>), dim3(grid),
-                           dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, plan.total[c], ctr, 2 * m);
-      else
-        hipLaunchKernelGGL((k_stream2_batch<2, kBatchWY, true>), dim3(grid),
-                           dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, plan.total[c], ctr, 2 * m);
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    (void)launch_zero((float*)ctr, 2 * (size_t)plan.chunk, cs);
+    for (int i = 0; i < plan.chunk; ++i) launch_pair(plan, cs, ctr + 2 * i, base, 2 * i);
+    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, cs, base, 2 * plan.chunk);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    if (e == hipSuccess) {
+      plan.graph = g;
+      e = hipGraphInstantiate(&plan.exec, g, nullptr, nullptr, 0);
     }
   }
+  (void)hipStreamDestroy(cs);
+  return e;
+}
+
+hipError_t batch2_pairs(Batch2Plan& plan, int m0, int m1, hipStream_t s) {
+  m1 = std::min(m1, plan.npairs);
+  if (m0 >= m1) return hipSuccess;
+  hipError_t e = hipSuccess;
+  if (plan.chunk > 0 && m1 - m0 >= plan.chunk) {
+    if (!plan.exec) e = build_chunk_graph(plan);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, chunk_base(plan), 2 * m0);
+    for (; m1 - m0 >= plan.chunk && e == hipSuccess; m0 += plan.chunk)
+      e = hipGraphLaunch(plan.exec, s);
+    if (e != hipSuccess || m0 >= m1) return e;
+  }
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.rare_bytes + plan.ts_bytes);
+  // Queue counters of these pairs start at zero (a fill kernel).
+  e = launch_zero((float*)(counters + 2 * m0), 2 * (size_t)(m1 - m0), s);
+  if (e != hipSuccess) return e;
+  for (int m = m0; m < m1; ++m) launch_pair(plan, s, counters + 2 * m, nullptr, 2 * m);
   return hipGetLastError();
+}
+
+hipError_t batch2_release(Batch2Plan& plan, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (plan.exec) {
+    // The graph may still be running: destroy it once the stream drained.
+    e = hipStreamSynchronize(s);
+    (void)hipGraphExecDestroy(plan.exec);
+    plan.exec = nullptr;
+  }
+  if (plan.graph) {
+    (void)hipGraphDestroy(plan.graph);
+    plan.graph = nullptr;
+  }
+  return e;
 }
 
 __global__ void k_capture_from(Stream2Rare* r, int n, int begin) {
@@ -863,8 +943,9 @@ hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s)
 hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) {
   Batch2Plan plan;
   hipError_t e = batch2_prepare(items, n, meta, plan, s);
-  if (e != hipSuccess) return e;
-  return batch2_pairs(plan, 0, plan.npairs, s);
+  if (e == hipSuccess) e = batch2_pairs(plan, 0, plan.npairs, s);
+  const hipError_t r = batch2_release(plan, s);
+  return e != hipSuccess ? e : r;
 }
 
 }  // namespace pjz

@@ -22,7 +22,7 @@ from . import _lib
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "reference": _lib.KERNEL_REFERENCE,
             "stream": _lib.KERNEL_STREAM, "stream2": _lib.KERNEL_STREAM2,
             "autotune": _lib.KERNEL_AUTOTUNE}
-_LAUNCH_KEYS = {"kernel", "tile_x", "tile_y", "waves_per_simd", "ckpt_every"}
+_LAUNCH_KEYS = {"kernel", "tile_x", "tile_y", "waves_per_simd", "ckpt_every", "graph_steps"}
 
 
 def _launch_options(launch_params):
@@ -32,7 +32,9 @@ def _launch_options(launch_params):
   first call per shape times the kernels and tilings, later calls reuse the
   fastest), ``tile_x``, ``tile_y``,
   ``waves_per_simd`` (stream kernel tiling), ``ckpt_every`` (steps between
-  checkpoints kept for the gradient; default about sqrt(2 * steps)).
+  checkpoints kept for the gradient; default about sqrt(2 * steps)),
+  ``graph_steps`` (batches: steps per captured and replayed hipGraph chunk,
+  even, <= 128; ``fdtdz_batch`` defaults to 64, 0 launches every step pair).
   """
   if launch_params is None:
     return {}
@@ -111,6 +113,7 @@ def make_problem(epsilon_shape, dt, source_shape, source_position, mask_shape,
   p.tile_y = int(opts.get("tile_y", 0))
   p.waves_per_simd = int(opts.get("waves_per_simd", 0))
   p.ckpt_every = int(opts.get("ckpt_every", 0))
+  p.graph_steps = int(opts.get("graph_steps", 0))
   return p
 
 
@@ -356,6 +359,8 @@ def fdtdz_batch(calls):
   if len(calls) == 0:
     return []
   probs, devs, device = _batch_problems(calls)
+  if not any("graph_steps" in dict(_launch_options(kw.get("launch_params"))) for kw in calls):
+    probs[0].graph_steps = 64
   epsilons = [kw["epsilon"] for kw in calls]
   wants_grad = torch.is_grad_enabled() and any(
       isinstance(e, torch.Tensor) and e.requires_grad for e in epsilons)
