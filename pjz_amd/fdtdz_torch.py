@@ -34,7 +34,7 @@ def _launch_options(launch_params):
   ``waves_per_simd`` (stream kernel tiling), ``ckpt_every`` (steps between
   checkpoints kept for the gradient; default about sqrt(2 * steps)),
   ``graph_steps`` (batches: steps per captured and replayed hipGraph chunk,
-  even, <= 128; ``fdtdz_batch`` defaults to 64, 0 launches every step pair).
+  even, <= 128; 0, the default, launches every step pair directly).
   """
   if launch_params is None:
     return {}
@@ -359,8 +359,6 @@ def fdtdz_batch(calls):
   if len(calls) == 0:
     return []
   probs, devs, device = _batch_problems(calls)
-  if not any("graph_steps" in dict(_launch_options(kw.get("launch_params"))) for kw in calls):
-    probs[0].graph_steps = 64
   epsilons = [kw["epsilon"] for kw in calls]
   wants_grad = torch.is_grad_enabled() and any(
       isinstance(e, torch.Tensor) and e.requires_grad for e in epsilons)
