@@ -609,6 +609,9 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
   // The reference kernel choice also selects the cell-by-cell reverse step.
   const bool stream_adj =
       p->kernel != PJZ_FDTD_KERNEL_REFERENCE && pjz::adjoint_stream_supported(g);
+  // Two reversed steps per launch unless the one-step kernels were asked for.
+  const bool stream2_adj = stream_adj && p->kernel != PJZ_FDTD_KERNEL_STREAM &&
+                           pjz::adjoint_stream2_supported(g);
   pjz::Src src{p->src_axis, p->src_pos, p->src_quad, source_field, source_waveform};
   pjz::OutWin nowin{p->x0, p->y0, p->z0, p->ex, p->ey, p->ez, nullptr};
   float* seg = (float*)(ws + L.seg);
@@ -622,17 +625,28 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
     e = restore_state(g, L, ws, k, s);
     if (e == hipSuccess)
       e = run_steps(g, *p, absorption_mask, src, nowin, ws, L, b, end, &cur, seg, s);
-    for (int n = end - 1; n >= b && e == hipSuccess; --n) {
-      if (n >= p->out_start && n < p->out_stop && (n - p->out_start) % p->out_step == 0)
+    auto is_out = [&](int n) {
+      return n >= p->out_start && n < p->out_stop && (n - p->out_start) % p->out_step == 0;
+    };
+    for (int n = end - 1; n >= b && e == hipSuccess;) {
+      if (is_out(n))
         e = pjz::launch_inject(g, *p, (float*)(ws + L.lE_[acur]),
                                grad_out + gstride * ((n - p->out_start) / p->out_step), s);
       if (e != hipSuccess) break;
-      if (stream_adj) {
+      if (stream2_adj && n - 1 >= b && !is_out(n - 1)) {
+        // Steps n and n-1 in one launch (no cotangent enters between them).
+        e = pjz::launch_adjoint_stream2(g, *p, absorption_mask, ws, L, seg + (n - b) * f3,
+                                        seg + (n - 1 - b) * f3, acur, s);
+        acur = 1 - acur;
+        n -= 2;
+      } else if (stream_adj) {
         e = pjz::launch_adjoint_stream(g, *p, absorption_mask, ws, L, seg + (n - b) * f3, acur,
                                        s);
         acur = 1 - acur;
+        --n;
       } else {
         e = pjz::launch_adjoint_step(g, *p, absorption_mask, ws, L, seg + (n - b) * f3, acur, s);
+        --n;
       }
     }
   }

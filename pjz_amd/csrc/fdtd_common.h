@@ -194,6 +194,13 @@ hipError_t launch_adjoint_step(const Geom& g, const pjz_fdtd_problem& p, const f
                                char* ws, const Layout& L, const float* D, int acur,
                                hipStream_t s);
 
+// Two reversed steps n, n-1 (D1 = D^n, D2 = D^(n-1)) in one launch: reads
+// adjoint set `acur`, writes set 1 - acur.
+bool adjoint_stream2_supported(const Geom& g);
+hipError_t launch_adjoint_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                                  char* ws, const Layout& L, const float* D1, const float* D2,
+                                  int acur, hipStream_t s);
+
 // The same reverse step as one streaming launch: reads adjoint set `acur`,
 // writes set 1 - acur.
 bool adjoint_stream_supported(const Geom& g);

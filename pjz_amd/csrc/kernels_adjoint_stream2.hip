@@ -1,0 +1,496 @@
+// Two reversed steps per launch: temporal blocking of depth 2 for the
+// adjoint, the counterpart of the two-step forward stepper.  Level 1 reverses
+// step n (D^n) at plane x while level 2 reverses step n-1 (D^(n-1)) one plane
+// behind at x-1, each with the row/plane dependencies of the one-step
+// streaming reverse kernel (kernels_adjoint_stream.hip):
+//   A: level-1 lH(x)   needs lK = cb lE at x+1 (prefetch) and y+1 (LDS sK1,
+//                      published one plane early from the prefetch)
+//   B: level-1 lE(x)   needs level-1 lH at y-1 (LDS sH1) and x-1 (registers)
+//      level-2 lH(x-1) needs level-1 lK at x (registers) and y+1 (LDS sK2,
+//                      published one plane earlier)
+//   C: level-2 lE(x-1) needs level-2 lH at y-1 (LDS sH2) and x-2 (registers)
+// Rows y0-2..y0+WY (WY+3 waves): level-1 lH on all, level-1 lE from y0-1,
+// level-2 lH on y0-1..y0+WY-1, level-2 lE (stored) on the owned rows; planes
+// xa-2, xa-1 are ghost planes.  lE, lH and both psi cotangents ping-pong
+// between the two adjoint sets; G is read and written only by the owner:
+//   G += lE_in D^n + lE_1 D^(n-1)
+// (lE_in the cotangent entering the pair, lE_1 after level 1).
+//
+// Per cell and pair: read lE, cb, lH, G, D^n, D^(n-1), write lE, lH, G
+// (108 B, + psi cotangents on PML cells) against 192 B for two one-step
+// launches.
+#include <algorithm>
+
+#include "stream_common.h"
+
+namespace pjz {
+
+struct Adj2Args {
+  const float* lEin;
+  const float* lHin;
+  float* lEout;
+  float* lHout;
+  const float* luin;  // psi_H cotangent pair (x at +0, y at +P)
+  float* luout;
+  const float* lpin;  // psi_E cotangent pair
+  float* lpout;
+  float* G;
+  const float* D1;  // D^n (level 1)
+  const float* D2;  // D^(n-1) (level 2)
+  const float* cb;
+  const float* zt;
+  const float* mask;
+  int X, Y, Z, lo, hi, NP, N, P;
+  float dt;
+  int LX, nty, ntiles;
+};
+
+template <int CPL>
+struct AdjCoef {
+  float ikE[CPL], ikEp[CPL], bE[CPL], cE[CPL], ikH[CPL], ikHm[CPL], bH[CPL], cH[CPL];
+  bool pml[CPL];
+  int pidx[CPL];
+};
+
+// Adjoint of the H half step for one row: lH_out, psi_H cotangent.
+//   K: lK at x; Kx: lK y, z at x+1; Ky: lK x, z at y+1 (same row's cells).
+template <int CPL>
+__device__ inline void adj_h_row(float (&h)[3][CPL], float (&lun)[2][CPL],
+                                 const float (&K)[3][CPL], const float (&Kx)[2][CPL],
+                                 const float (&Ky)[2][CPL], const float (&lh)[3][CPL],
+                                 const float (&lu)[2][CPL], const float (&lp)[2][CPL],
+                                 const AdjCoef<CPL>& zc, int zb, int Z, float dt, bool valid) {
+  float kxz[CPL], kyz[CPL], vx[CPL], vy[CPL], vxz[CPL], vyz[CPL];
+  z_next<CPL>(kxz, K[0], zb, Z);
+  z_next<CPL>(kyz, K[1], zb, Z);
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    vx[c] = zc.pml[c] ? zc.cE[c] * (lp[0][c] - K[0][c]) : 0.0f;
+    vy[c] = zc.pml[c] ? zc.cE[c] * (lp[1][c] + K[1][c]) : 0.0f;
+  }
+  z_next<CPL>(vxz, vx, zb, Z);
+  z_next<CPL>(vyz, vy, zb, Z);
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    h[0][c] = lh[0][c] - (zc.ikEp[c] * kyz[c] - zc.ikE[c] * K[1][c]) + (Ky[1][c] - K[2][c]) -
+              (vyz[c] - vy[c]);
+    h[1][c] = lh[1][c] + (zc.ikEp[c] * kxz[c] - zc.ikE[c] * K[0][c]) - (Kx[1][c] - K[2][c]) -
+              (vxz[c] - vx[c]);
+    h[2][c] = lh[2][c] - (Ky[0][c] - K[0][c]) + (Kx[0][c] - K[1][c]);
+    if (!valid) h[0][c] = h[1][c] = h[2][c] = 0.0f;
+    lun[0][c] = zc.pml[c] ? zc.bH[c] * lu[0][c] + dt * h[0][c] : 0.0f;
+    lun[1][c] = zc.pml[c] ? zc.bH[c] * lu[1][c] - dt * h[1][c] : 0.0f;
+  }
+}
+
+// Adjoint of the E half step for one row: lE_out, psi_E cotangent.
+//   hdn: lH x, z at y-1; hp: lH y, z at x-1; le: lE entering the step.
+template <int CPL>
+__device__ inline void adj_e_row(float (&len)[3][CPL], float (&lpn)[2][CPL],
+                                 const float (&h)[3][CPL], const float (&hdn)[2][CPL],
+                                 const float (&hp)[2][CPL], const float (&lun)[2][CPL],
+                                 const float (&le)[3][CPL], const float (&cb)[3][CPL],
+                                 const float (&sig)[3], const float (&lp)[2][CPL],
+                                 const AdjCoef<CPL>& zc, int lane, float dt) {
+  float hxz[CPL], hyz[CPL], ux[CPL], uy[CPL], uxz[CPL], uyz[CPL];
+  z_prev<CPL>(hxz, h[0], lane);
+  z_prev<CPL>(hyz, h[1], lane);
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    ux[c] = zc.pml[c] ? zc.cH[c] * lun[0][c] : 0.0f;
+    uy[c] = zc.pml[c] ? zc.cH[c] * lun[1][c] : 0.0f;
+  }
+  z_prev<CPL>(uxz, ux, lane);
+  z_prev<CPL>(uyz, uy, lane);
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const float gx = dt * ((zc.ikH[c] * h[1][c] - zc.ikHm[c] * hyz[c]) - (h[2][c] - hdn[1][c])) -
+                     (uy[c] - uyz[c]);
+    const float gy = dt * (-(zc.ikH[c] * h[0][c] - zc.ikHm[c] * hxz[c]) + (h[2][c] - hp[1][c])) -
+                     (ux[c] - uxz[c]);
+    const float gz = dt * ((h[0][c] - hdn[0][c]) - (h[1][c] - hp[0][c]));
+    len[0][c] = (1.0f - sig[0] * cb[0][c]) * le[0][c] + gx;
+    len[1][c] = (1.0f - sig[1] * cb[1][c]) * le[1][c] + gy;
+    len[2][c] = (1.0f - sig[2] * cb[2][c]) * le[2][c] + gz;
+    lpn[0][c] = zc.pml[c] ? zc.bE[c] * (lp[0][c] - cb[0][c] * le[0][c]) : 0.0f;
+    lpn[1][c] = zc.pml[c] ? zc.bE[c] * (lp[1][c] + cb[1][c] * le[1][c]) : 0.0f;
+  }
+}
+
+// Level-1 inputs of one row, fetched one plane ahead.
+template <int CPL>
+struct Adj2Plane {
+  float le[3][CPL];   // lE at x + 1
+  float cb[3][CPL];   // cb at x + 1
+  float lh[3][CPL];   // lH at x
+  float lu[2][CPL];   // psi_H cotangent at x
+  float lp[2][CPL];   // psi_E cotangent at x
+  float kup[2][CPL];  // lK x, z at (x, y+1) (top row only)
+  float sig[3];       // absorber at (x, y)
+};
+
+// Carried from plane x-1 to plane x.
+template <int CPL>
+struct Adj2Carry {
+  float le0[3][CPL];  // lE entering the pair at x
+  float cb0[3][CPL];  // cb at x
+  float e1m[3][CPL];  // level-1 lE at x-1
+  float cbm[3][CPL];  // cb at x-1
+  float h1m[3][CPL];  // level-1 lH at x-1
+  float lu1m[2][CPL]; // level-1 psi_H cotangent at x-1
+  float lp1m[2][CPL]; // level-1 psi_E cotangent at x-1
+  float h2p[2][CPL];  // level-2 lH y, z at x-2
+  float gpm[3][CPL];  // lE_in D^n at x-1 (owned rows)
+  float sigm[3];
+};
+
+template <int CPL>
+__device__ inline void adj2_fetch(Adj2Plane<CPL>& P, const Adj2Args& a, int x, int yc, bool top,
+                                  int zb, bool active, const AdjCoef<CPL>& zc) {
+  const int N = a.N, Y = a.Y, Z = a.Z;
+  const int o = (x * Y + yc) * Z;
+  const bool nx = active && x + 1 < a.X;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    load_col<CPL, true>(P.le[k], a.lEin + (k * N + o + Y * Z), zb, Z, nx);
+    load_col<CPL, true>(P.cb[k], a.cb + (k * N + o + Y * Z), zb, Z, nx);
+    load_col<CPL, true>(P.lh[k], a.lHin + (k * N + o), zb, Z, active);
+  }
+  if (top) {
+    const bool v = active && yc + 1 < Y;
+    float e0[CPL], e2[CPL], c0[CPL], c2[CPL];
+    load_col<CPL, true>(e0, a.lEin + (o + Z), zb, Z, v);
+    load_col<CPL, true>(e2, a.lEin + (2 * N + o + Z), zb, Z, v);
+    load_col<CPL, true>(c0, a.cb + (o + Z), zb, Z, v);
+    load_col<CPL, true>(c2, a.cb + (2 * N + o + Z), zb, Z, v);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      P.kup[0][c] = c0[c] * e0[c];
+      P.kup[1][c] = c2[c] * e2[c];
+    }
+  }
+  const int pb = (x * Y + yc) * a.NP;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const bool ph = zc.pml[c] && active;
+    const int pi = pb + zc.pidx[c];
+    P.lu[0][c] = ph ? a.luin[pi] : 0.0f;
+    P.lu[1][c] = ph ? a.luin[a.P + pi] : 0.0f;
+    P.lp[0][c] = ph ? a.lpin[pi] : 0.0f;
+    P.lp[1][c] = ph ? a.lpin[a.P + pi] : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    P.sig[k] = active ? uniform_load(a.mask, (k * a.X + x) * a.Y + yc) : 0.0f;
+}
+
+// Row-exchange buffers (cell c of lane l at [c * 64 + l]).
+template <int CPL, int WY>
+struct Adj2Lds {
+  float sK1[2][WY + 3][2][64 * CPL];  // level-1 lK x, z (slot = plane parity)
+  float sH1[WY + 3][2][64 * CPL];     // level-1 lH x, z of plane x
+  float sK2[2][WY + 3][2][64 * CPL];  // level-1 lE turned lK for level 2, per plane parity
+  float sH2[WY + 3][2][64 * CPL];     // level-2 lH x, z of plane x-1
+};
+
+template <int CPL, int WY>
+__device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& L,
+                                           Adj2Carry<CPL>& s, const Adj2Plane<CPL>& cur,
+                                           Adj2Plane<CPL>& nxt, const AdjCoef<CPL>& zc, int x,
+                                           int xa, int xb, int w, int y, int yc, int lane,
+                                           int zb, bool row_valid, int PAR) {
+  constexpr int NW = WY + 3;
+  const bool e_rows = w >= 1;
+  const bool h2_rows = w >= 1 && w <= WY + 1;
+  const bool own = w >= 2 && w <= WY + 1;
+  const bool l1 = x < a.X;
+  const bool l2 = x >= xa && x >= 1;
+  const bool act = row_valid && l1;
+  const int N = a.N, Y = a.Y, Z = a.Z;
+  const float dt = a.dt;
+  if (x + 1 <= xb) adj2_fetch<CPL>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, zc);
+  // This plane's D^n (level 1 at x) and G, D^(n-1) (level 2 at x-1), owned
+  // rows only: issued now, used in phases B and C.
+  float d1[3][CPL], g[3][CPL], d2[3][CPL];
+  {
+    const int o = (x * Y + yc) * Z, om = ((x - 1) * Y + yc) * Z;
+    const bool st = own && row_valid && l2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      load_col<CPL, true>(d1[k], a.D1 + (k * N + o), zb, Z, own && act);
+      load_col<CPL, true>(g[k], a.G + (k * N + om), zb, Z, st);
+      load_col<CPL, true>(d2[k], a.D2 + (k * N + om), zb, Z, st);
+    }
+  }
+
+  // ---- A: level-1 lH at plane x
+  float K1[3][CPL];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) K1[k][c] = s.cb0[k][c] * s.le0[k][c];
+  float Kx[2][CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    Kx[0][c] = cur.cb[1][c] * cur.le[1][c];
+    Kx[1][c] = cur.cb[2][c] * cur.le[2][c];
+    // lK x, z of plane x+1 for the row below, read next plane.
+    L.sK1[PAR ^ 1][w][0][c * 64 + lane] = cur.cb[0][c] * cur.le[0][c];
+    L.sK1[PAR ^ 1][w][1][c * 64 + lane] = Kx[1][c];
+  }
+  float Ky[2][CPL];
+  if (w < NW - 1) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      Ky[0][c] = L.sK1[PAR][w + 1][0][c * 64 + lane];
+      Ky[1][c] = L.sK1[PAR][w + 1][1][c * 64 + lane];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      Ky[0][c] = cur.kup[0][c];
+      Ky[1][c] = cur.kup[1][c];
+    }
+  }
+  float h1[3][CPL], lu1[2][CPL];
+  adj_h_row<CPL>(h1, lu1, K1, Kx, Ky, cur.lh, cur.lu, cur.lp, zc, zb, Z, dt, act);
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    L.sH1[w][0][c * 64 + lane] = h1[0][c];
+    L.sH1[w][1][c * 64 + lane] = h1[2][c];
+  }
+  lds_barrier();
+
+  // ---- B: level-1 lE at plane x, level-2 lH at plane x-1
+  float e1[3][CPL], lp1[2][CPL];
+  if (e_rows) {
+    float hdn[2][CPL], hp[2][CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      hdn[0][c] = L.sH1[w - 1][0][c * 64 + lane];
+      hdn[1][c] = L.sH1[w - 1][1][c * 64 + lane];
+      hp[0][c] = s.h1m[1][c];
+      hp[1][c] = s.h1m[2][c];
+    }
+    adj_e_row<CPL>(e1, lp1, h1, hdn, hp, lu1, s.le0, s.cb0, cur.sig, cur.lp, zc, lane, dt);
+    if (!act) {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) e1[0][c] = e1[1][c] = e1[2][c] = lp1[0][c] = lp1[1][c] = 0.0f;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) e1[0][c] = e1[1][c] = e1[2][c] = lp1[0][c] = lp1[1][c] = 0.0f;
+  }
+  // lE_in D^n at x (owned rows), added to G with level 2 next plane.
+  float gp[3][CPL];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) gp[k][c] = s.le0[k][c] * d1[k][c];
+  float K2x[2][CPL];  // level-2 lK y, z at x (this row)
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    K2x[0][c] = s.cb0[1][c] * e1[1][c];
+    K2x[1][c] = s.cb0[2][c] * e1[2][c];
+    L.sK2[PAR][w][0][c * 64 + lane] = s.cb0[0][c] * e1[0][c];
+    L.sK2[PAR][w][1][c * 64 + lane] = K2x[1][c];
+  }
+  float h2[3][CPL], lu2[2][CPL];
+  if (h2_rows && l2) {
+    float K2[3][CPL], K2y[2][CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) K2[k][c] = s.cbm[k][c] * s.e1m[k][c];
+      K2y[0][c] = L.sK2[PAR ^ 1][w + 1][0][c * 64 + lane];
+      K2y[1][c] = L.sK2[PAR ^ 1][w + 1][1][c * 64 + lane];
+    }
+    adj_h_row<CPL>(h2, lu2, K2, K2x, K2y, s.h1m, s.lu1m, s.lp1m, zc, zb, Z, dt, row_valid);
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) h2[0][c] = h2[1][c] = h2[2][c] = lu2[0][c] = lu2[1][c] = 0.0f;
+  }
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    L.sH2[w][0][c * 64 + lane] = h2[0][c];
+    L.sH2[w][1][c * 64 + lane] = h2[2][c];
+  }
+  lds_barrier();
+
+  // ---- C: level-2 lE at plane x-1 (owned rows), stores
+  if (own && l2 && row_valid && x - 1 >= xa) {
+    float hdn[2][CPL], e2[3][CPL], lp2[2][CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      hdn[0][c] = L.sH2[w - 1][0][c * 64 + lane];
+      hdn[1][c] = L.sH2[w - 1][1][c * 64 + lane];
+    }
+    adj_e_row<CPL>(e2, lp2, h2, hdn, s.h2p, lu2, s.e1m, s.cbm, s.sigm, s.lp1m, zc, lane, dt);
+    const int o = ((x - 1) * Y + y) * Z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float gn[CPL];
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) gn[c] = g[k][c] + s.gpm[k][c] + s.e1m[k][c] * d2[k][c];
+      store_col<CPL, true>(a.lEout + (k * N + o), e2[k], zb, Z);
+      store_col<CPL, true>(a.lHout + (k * N + o), h2[k], zb, Z);
+      store_col<CPL, true>(a.G + (k * N + o), gn, zb, Z);
+    }
+    const int pb = ((x - 1) * Y + y) * a.NP;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      if (zc.pml[c]) {
+        const int pi = pb + zc.pidx[c];
+        a.luout[pi] = lu2[0][c];
+        a.luout[a.P + pi] = lu2[1][c];
+        a.lpout[pi] = lp2[0][c];
+        a.lpout[a.P + pi] = lp2[1][c];
+      }
+    }
+  }
+
+  // ---- carry to plane x+1
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    s.h2p[0][c] = h2[1][c];
+    s.h2p[1][c] = h2[2][c];
+    s.lu1m[0][c] = lu1[0][c];
+    s.lu1m[1][c] = lu1[1][c];
+    s.lp1m[0][c] = lp1[0][c];
+    s.lp1m[1][c] = lp1[1][c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      s.e1m[k][c] = e1[k][c];
+      s.h1m[k][c] = h1[k][c];
+      s.cbm[k][c] = s.cb0[k][c];
+      s.gpm[k][c] = gp[k][c];
+      s.le0[k][c] = cur.le[k][c];
+      s.cb0[k][c] = cur.cb[k][c];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) s.sigm[k] = cur.sig[k];
+}
+
+template <int CPL, int WY>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2(Adj2Args a) {
+  __shared__ Adj2Lds<CPL, WY> L;
+  constexpr int NW = WY + 3;
+  int tile = blockIdx.x;
+  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
+  const int Z = a.Z;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int zb = lane * CPL;
+  const int ty = tile % a.nty, tx = tile / a.nty;
+  const int y = ty * WY - 2 + w;
+  const bool row_valid = y >= 0 && y < a.Y;
+  const int yc = min(max(y, 0), a.Y - 1);
+  const int xa = tx * a.LX;
+  const int xb = min(xa + a.LX, a.X);
+
+  AdjCoef<CPL> zc;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const int z = zb + c;
+    const int zz = min(z, Z - 1);
+    zc.ikE[c] = a.zt[IKE * Z + zz];
+    zc.ikEp[c] = z + 1 < Z ? a.zt[IKE * Z + z + 1] : 0.0f;
+    zc.bE[c] = a.zt[BE * Z + zz];
+    zc.cE[c] = a.zt[CE * Z + zz];
+    zc.ikH[c] = a.zt[IKH * Z + zz];
+    zc.ikHm[c] = z >= 1 && z - 1 < Z ? a.zt[IKH * Z + z - 1] : 0.0f;
+    zc.bH[c] = a.zt[BH * Z + zz];
+    zc.cH[c] = a.zt[CH * Z + zz];
+    zc.pml[c] = z < Z && (z < a.lo || z >= Z - a.hi);
+    zc.pidx[c] = z < a.lo ? z : z - (Z - a.hi) + a.lo;
+  }
+
+  Adj2Carry<CPL> s;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s.e1m[k][c] = s.cbm[k][c] = s.h1m[k][c] = s.gpm[k][c] = 0.0f;
+    s.lu1m[0][c] = s.lu1m[1][c] = s.lp1m[0][c] = s.lp1m[1][c] = 0.0f;
+    s.h2p[0][c] = s.h2p[1][c] = 0.0f;
+  }
+  s.sigm[0] = s.sigm[1] = s.sigm[2] = 0.0f;
+  const int xs = max(xa - 2, 0);
+  {
+    const int o = (xs * a.Y + yc) * Z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      load_col<CPL, true>(s.le0[k], a.lEin + (k * a.N + o), zb, Z, row_valid);
+      load_col<CPL, true>(s.cb0[k], a.cb + (k * a.N + o), zb, Z, row_valid);
+    }
+  }
+  Adj2Plane<CPL> cur, nxt;
+  adj2_fetch<CPL>(cur, a, xs, yc, w == NW - 1, zb, row_valid, zc);
+  // lK x, z of the first plane for the row below.
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    L.sK1[0][w][0][c * 64 + lane] = s.cb0[0][c] * s.le0[0][c];
+    L.sK1[0][w][1][c * 64 + lane] = s.cb0[2][c] * s.le0[2][c];
+  }
+  lds_barrier();
+  int par = 0;
+  for (int x = xs; x <= xb; ++x) {
+    adj2_plane<CPL, WY>(a, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid, par);
+    cur = nxt;
+    par ^= 1;
+  }
+}
+
+// Whether the two-step reverse kernel handles this problem.
+bool adjoint_stream2_supported(const Geom& g) {
+  return (g.Z <= 64) || (g.Z <= 128 && g.Z % 2 == 0);
+}
+
+hipError_t launch_adjoint_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                                  char* ws, const Layout& L, const float* D1, const float* D2,
+                                  int acur, hipStream_t s) {
+  if (!adjoint_stream2_supported(g)) return hipErrorInvalidValue;
+  Adj2Args a;
+  const int in = acur ? 1 : 0, out = 1 - in;
+  a.lEin = (const float*)(ws + L.lE_[in]);
+  a.lHin = (const float*)(ws + L.lH_[in]);
+  a.luin = (const float*)(ws + L.lu_[in]);
+  a.lpin = (const float*)(ws + L.lpE_[in]);
+  a.lEout = (float*)(ws + L.lE_[out]);
+  a.lHout = (float*)(ws + L.lH_[out]);
+  a.luout = (float*)(ws + L.lu_[out]);
+  a.lpout = (float*)(ws + L.lpE_[out]);
+  a.G = (float*)(ws + L.accG);
+  a.D1 = D1;
+  a.D2 = D2;
+  a.cb = (const float*)(ws + L.cb);
+  a.zt = (const float*)(ws + L.zt);
+  a.mask = mask;
+  a.X = g.X;
+  a.Y = g.Y;
+  a.Z = g.Z;
+  a.lo = g.lo;
+  a.hi = g.hi;
+  a.NP = g.NP;
+  a.N = (int)g.N;
+  a.P = g.X * g.Y * g.NP;
+  a.dt = p.dt;
+  // 4 rows per tile (7 waves): the two levels' carried state needs ~200
+  // registers a lane, which 2 waves per SIMD allow without spills (8 rows,
+  // 11 waves, would cap it at 168 and spill: measured slower).
+  constexpr int WY = 4;
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  a.nty = (g.Y + WY - 1) / WY;
+  const int ntx = std::max(1, (cus + a.nty - 1) / a.nty);
+  a.LX = std::max(8, (g.X + ntx - 1) / ntx);
+  a.ntiles = a.nty * ((g.X + a.LX - 1) / a.LX);
+  if (g.Z <= 64)
+    hipLaunchKernelGGL((k_adj_stream2<1, WY>), dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_adj_stream2<2, WY>), dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace pjz

@@ -65,6 +65,18 @@ def test_gradient_matches_oracle(kernel, axis):
   assert err <= TOL
 
 
+@pytest.mark.parametrize("tile_y", [4, 8])
+@pytest.mark.parametrize("axis", [0, 2])
+def test_two_step_reverse_kernel_tiles(axis, tile_y):
+  """The two-step reverse kernel (default with the two-step forward) in both
+  row tilings, z = 64 and 128 lanes layouts, against the oracle."""
+  for Z in (24, 80):
+    eps, kw, g = problem(axis, seed=40 + axis, Z=Z)
+    ref = adjoint.grad_epsilon(eps, g_out=g, **kw)
+    got, _ = gpu_grad(eps, kw, g, kernel="stream2", tile_y=tile_y, ckpt_every=30)
+    assert rel_l2(got, ref) <= TOL, (Z, tile_y)
+
+
 @pytest.mark.parametrize("kernel", ["stream", "stream2"])
 @pytest.mark.parametrize("ckpt", [1, 7, 88, 1000])
 def test_gradient_independent_of_checkpoint_spacing(ckpt, kernel):
