@@ -359,13 +359,19 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
 
 namespace {
 
+int max_steps(const pjz_fdtd_problem* probs, int n) {
+  int m = 0;
+  for (int i = 0; i < n; ++i) m = std::max(m, pjz::steps_of(probs[i]));
+  return m;
+}
+
 // Workspace plan of a batch: one prepared single-simulation layout per item
 // (with checkpoints and adjoint buffers when ckpt_every > 0), then the batch
 // tables.
 struct BatchPlanHost {
   std::vector<pjz::Layout> Ls;
   std::vector<size_t> offs;
-  size_t meta_off = 0, adj_off = 0, total = 0;
+  size_t meta_off = 0, adj_off = 0, adj2_off = 0, total = 0;
   int S = 0;  // common checkpoint spacing, 0 = forward only
 };
 
@@ -401,7 +407,8 @@ int batch_plan(const pjz_fdtd_problem* probs, int n, BatchPlanHost& bp) {
   for (int i = 0; i < n; ++i)
     items[i] = {geom_of(probs[i]), &probs[i], nullptr, {}, {}, nullptr, &bp.Ls[i], nullptr};
   bp.adj_off = off + pjz::align256(pjz::batch2_meta_bytes(items.data(), n));
-  bp.total = bp.adj_off + (bp.S > 0 ? pjz::adjoint_batch_meta_bytes(n, bp.S) : 0);
+  bp.adj2_off = bp.adj_off + (bp.S > 0 ? pjz::align256(pjz::adjoint_batch_meta_bytes(n, bp.S)) : 0);
+  bp.total = bp.adj2_off + (bp.S > 0 ? pjz::adjoint2_batch_meta_bytes(n, bp.S) : 0);
   return PJZ_FDTD_OK;
 }
 
@@ -523,41 +530,82 @@ int pjz_fdtd_batch_backward(const pjz_fdtd_problem* probs, int n, const float* c
                    items);
   if (rc) return rc;
   char* base = (char*)workspace;
+  const int last_seg = (std::max(1, max_steps(probs, n)) - 1) / bp.S;
+  auto seg_end = [&](int k) {
+    int end = k * bp.S;
+    for (int i = 0; i < n; ++i)
+      if (k < bp.Ls[i].nck) end = std::max(end, std::min((k + 1) * bp.S, pjz::steps_of(probs[i])));
+    return end;
+  };
+  auto is_out = [&](int i, int m) {
+    const pjz_fdtd_problem& q = probs[i];
+    return m < pjz::steps_of(q) && m >= q.out_start && m < q.out_stop &&
+           (m - q.out_start) % q.out_step == 0;
+  };
+  // The reverse schedule, fixed up front: launches of one or two reversed
+  // steps in decreasing step order, segment by segment.  A pair (m, m-1) is
+  // used unless a snapshot cotangent enters some simulation at m-1 or some
+  // simulation's first reversed step is m-1.  lstart[i]: the launch a
+  // simulation joins at (its adjoint set is (launch - lstart) & 1).
+  struct RevLaunch {
+    int m, b;
+    bool two;
+  };
+  std::vector<RevLaunch> sched;
+  std::vector<int> lstart(n, 0);
+  std::vector<bool> joined(n, false);
+  for (int k = last_seg; k >= 0; --k) {
+    const int b = k * bp.S;
+    for (int m = seg_end(k) - 1; m >= b;) {
+      bool two = m - 1 >= b;
+      for (int i = 0; i < n && two; ++i)
+        if (pjz::steps_of(probs[i]) == m || is_out(i, m - 1)) two = false;
+      for (int i = 0; i < n; ++i)
+        if (!joined[i] && m < pjz::steps_of(probs[i])) {
+          joined[i] = true;
+          lstart[i] = (int)sched.size();
+        }
+      sched.push_back({m, b, two});
+      m -= two ? 2 : 1;
+    }
+  }
   pjz::Batch2Plan plan;
   pjz::AdjBatchPlan aplan;
+  pjz::Adj2BatchPlan a2plan;
   hipError_t e = pjz::batch2_prepare(items.data(), n, base + bp.meta_off, plan, s);
   if (e == hipSuccess)
-    e = pjz::adjoint_batch_prepare(items.data(), n, bp.S, base + bp.adj_off, aplan, s);
-  const int last_seg = (2 * plan.npairs - 1) / bp.S;
+    e = pjz::adjoint_batch_prepare(items.data(), n, bp.S, lstart.data(), base + bp.adj_off, aplan,
+                                   s);
+  if (e == hipSuccess)
+    e = pjz::adjoint2_batch_prepare(items.data(), n, bp.S, lstart.data(), base + bp.adj2_off,
+                                    a2plan, s);
+  size_t L = 0;
   for (int k = last_seg; k >= 0 && e == hipSuccess; --k) {
     const int b = k * bp.S;
     // Recompute segment k of every simulation that reaches it from its
     // checkpoint, capturing D^n = E^n - E^(n+1) (slot n - b).
-    int end = b;
     for (int i = 0; i < n && e == hipSuccess; ++i) {
-      const pjz::Layout& L = bp.Ls[i];
-      if (k >= L.nck) continue;
-      const int c = (b / 2) & 1;
-      end = std::max(end, std::min(b + bp.S, pjz::steps_of(probs[i])));
-      e = copy_state(items[i].g, L, items[i].ws, c, k, false, s);
+      const pjz::Layout& Li = bp.Ls[i];
+      if (k >= Li.nck) continue;
+      e = copy_state(items[i].g, Li, items[i].ws, (b / 2) & 1, k, false, s);
     }
     if (e == hipSuccess) e = pjz::batch2_capture_from(plan, b, s);
-    if (e == hipSuccess) e = pjz::batch2_pairs(plan, b / 2, (end + 1) / 2, s);
-    // Reverse steps end-1 .. b of the whole batch.  At reversed step m a
-    // simulation's cotangents sit in adjoint set (nsteps - 1 - m) & 1.
+    if (e == hipSuccess) e = pjz::batch2_pairs(plan, b / 2, (seg_end(k) + 1) / 2, s);
     if (e == hipSuccess) e = pjz::adjoint_batch_begin_segment(aplan, s);
-    for (int m = end - 1; m >= b && e == hipSuccess; --m) {
+    if (e == hipSuccess) e = pjz::adjoint2_batch_begin_segment(a2plan, s);
+    for (; L < sched.size() && sched[L].b == b && e == hipSuccess; ++L) {
+      const int m = sched[L].m;
       for (int i = 0; i < n && e == hipSuccess; ++i) {
+        if (!is_out(i, m)) continue;
         const pjz_fdtd_problem& q = probs[i];
-        const int ns = pjz::steps_of(q);
-        if (m < ns && m >= q.out_start && m < q.out_stop && (m - q.out_start) % q.out_step == 0) {
-          const long long gstride = 3LL * q.ex * q.ey * q.ez;
-          e = pjz::launch_inject(items[i].g, q,
-                                 (float*)(items[i].ws + bp.Ls[i].lE_[(ns - 1 - m) & 1]),
-                                 grad_out[i] + gstride * ((m - q.out_start) / q.out_step), s);
-        }
+        const long long gstride = 3LL * q.ex * q.ey * q.ez;
+        e = pjz::launch_inject(items[i].g, q,
+                               (float*)(items[i].ws + bp.Ls[i].lE_[((int)L - lstart[i]) & 1]),
+                               grad_out[i] + gstride * ((m - q.out_start) / q.out_step), s);
       }
-      if (e == hipSuccess) e = pjz::launch_adjoint_batch_step(aplan, m, b, s);
+      if (e != hipSuccess) break;
+      e = sched[L].two ? pjz::launch_adjoint2_batch_step(a2plan, (int)L, m, b, s)
+                       : pjz::launch_adjoint_batch_step(aplan, (int)L, m, b, s);
     }
   }
   for (int i = 0; i < n && e == hipSuccess; ++i)

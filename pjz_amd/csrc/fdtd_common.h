@@ -211,22 +211,33 @@ hipError_t launch_adjoint_stream(const Geom& g, const pjz_fdtd_problem& p, const
 // Batched reverse steps: one persistent streaming launch per cells-per-lane
 // class for every simulation of a batch, (simulation, tile) items taken from
 // a device queue.  Each item's D^n is in its `capture` buffer at slot
-// n - seg_begin; a simulation with nsteps steps reads adjoint set
-// (nsteps - 1 - n) & 1 at reversed step n (sets alternate per reversed step,
-// starting from set 0) and skips steps at or past its last.  S bounds the
-// steps of a segment; adjoint_batch_begin_segment clears the queue counters.
+// n - seg_begin; a simulation skips steps at or past its last.  The backward
+// is a fixed sequence of launches (one or two reversed steps each); a
+// simulation that joins at launch lstart[i] reads adjoint set
+// (launch - lstart[i]) & 1.  S bounds the steps of a segment;
+// *_begin_segment clears the queue counters.
 struct AdjBatchPlan {
   int n, S, cus;
   int count[2], base[2], total[2];
   char* meta;
-  size_t args_bytes, ts_bytes;
+  size_t args_bytes, ts_bytes, ls_bytes;
 };
 size_t adjoint_batch_meta_bytes(int n, int S);
-hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, char* meta,
-                                 AdjBatchPlan& plan, hipStream_t s);
+hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, const int* lstart,
+                                 char* meta, AdjBatchPlan& plan, hipStream_t s);
 hipError_t adjoint_batch_begin_segment(const AdjBatchPlan& plan, hipStream_t s);
-hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int step, int seg_begin,
-                                     hipStream_t s);
+hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int launch, int step,
+                                     int seg_begin, hipStream_t s);
+
+// Pairs of reversed steps (step, step-1) for a batch (the two-step reverse
+// kernel in persistent-queue form); same conventions.
+using Adj2BatchPlan = AdjBatchPlan;
+size_t adjoint2_batch_meta_bytes(int n, int S);
+hipError_t adjoint2_batch_prepare(const BatchItem* items, int n, int S, const int* lstart,
+                                  char* meta, Adj2BatchPlan& plan, hipStream_t s);
+hipError_t adjoint2_batch_begin_segment(const Adj2BatchPlan& plan, hipStream_t s);
+hipError_t launch_adjoint2_batch_step(const Adj2BatchPlan& plan, int launch, int step,
+                                      int seg_begin, hipStream_t s);
 
 // lE[window] += g (one snapshot cotangent, (3, ex, ey, ez)).
 hipError_t launch_inject(const Geom& g, const pjz_fdtd_problem& p, float* lE, const float* g_out,

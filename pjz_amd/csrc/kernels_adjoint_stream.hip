@@ -320,13 +320,14 @@ __global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream(AdjStreamArgs a) {
 // Batched reverse step: persistent workgroups take (simulation, tile) items
 // from a device counter, as the batched two-step forward kernel does.  args
 // holds two views per simulation (which adjoint set is read); a simulation
-// with nsteps steps has reversed nsteps-1-step steps before this one, which
-// fixes the set it reads.  D of the step sits at slot step - seg_begin of the
-// view's D base.
+// that joined the backward at launch lstart[s] reads set
+// (launch - lstart[s]) & 1.  D of the step sits at slot step - seg_begin of
+// the view's D base.
 template <int CPL, int WY, bool vec>
 __global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream_batch(
-    const AdjStreamArgs* __restrict__ args, const int* __restrict__ tile_start, int nsims,
-    int total, int* __restrict__ counter, int step, int seg_begin) {
+    const AdjStreamArgs* __restrict__ args, const int* __restrict__ tile_start,
+    const int* __restrict__ lstart, int nsims, int total, int* __restrict__ counter, int launch,
+    int step, int seg_begin) {
   __shared__ AdjLds<CPL, WY> S;
   __shared__ int s_item;
   for (;;) {
@@ -341,9 +342,8 @@ __global__ __launch_bounds__(64 * (WY + 1)) void k_adj_stream_batch(
       if (tile_start[mid] <= t) lo = mid;
       else hi = mid - 1;
     }
-    const int ns = args[2 * lo].nsteps;
-    if (step >= ns) continue;
-    AdjStreamArgs a = args[2 * lo + ((ns - 1 - step) & 1)];
+    if (step >= args[2 * lo].nsteps) continue;
+    AdjStreamArgs a = args[2 * lo + ((launch - lstart[lo]) & 1)];
     a.D += (long long)(step - seg_begin) * 3 * a.N;
     adj_tile<CPL, WY, vec>(a, t - tile_start[lo], S);
   }
@@ -426,11 +426,11 @@ static constexpr int kAdjBatchLX = 128;
 
 size_t adjoint_batch_meta_bytes(int n, int S) {
   return align256(sizeof(AdjStreamArgs) * 2 * (size_t)n) + align256(sizeof(int) * (size_t)(n + 2)) +
-         align256(sizeof(int) * 2 * (size_t)std::max(S, 1));
+         align256(sizeof(int) * (size_t)n) + align256(sizeof(int) * 2 * (size_t)std::max(S, 1));
 }
 
-hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, char* meta,
-                                 AdjBatchPlan& plan, hipStream_t s) {
+hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, const int* lstart,
+                                 char* meta, AdjBatchPlan& plan, hipStream_t s) {
   std::vector<int> cls[2];
   for (int i = 0; i < n; ++i) {
     if (!adjoint_stream_supported(items[i].g) || items[i].g.Z > 128) return hipErrorInvalidValue;
@@ -442,9 +442,11 @@ hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, char* met
   plan.cus = device_cus();
   plan.args_bytes = align256(sizeof(AdjStreamArgs) * 2 * (size_t)n);
   plan.ts_bytes = align256(sizeof(int) * (size_t)(n + 2));
-  std::vector<char> host(plan.args_bytes + plan.ts_bytes, 0);
+  plan.ls_bytes = align256(sizeof(int) * (size_t)n);
+  std::vector<char> host(plan.args_bytes + plan.ts_bytes + plan.ls_bytes, 0);
   AdjStreamArgs* hargs = (AdjStreamArgs*)host.data();
   int* hts = (int*)(host.data() + plan.args_bytes);
+  int* hls = (int*)(host.data() + plan.args_bytes + plan.ts_bytes);
   int off = 0;
   for (int c = 0; c < 2; ++c) {
     plan.base[c] = off;
@@ -457,6 +459,7 @@ hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, char* met
         hargs[2 * (off + k) + v] =
             adj_stream_args(it.g, *it.p, it.mask, it.ws, *it.L, it.capture, v, LX);
       hts[off + c + k] = t;
+      hls[off + k] = lstart[cls[c][k]];
       t += hargs[2 * (off + k)].ntiles;
     }
     hts[off + c + cls[c].size()] = t;
@@ -470,30 +473,32 @@ hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, char* met
 }
 
 hipError_t adjoint_batch_begin_segment(const AdjBatchPlan& plan, hipStream_t s) {
-  int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes);
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes + plan.ls_bytes);
   return launch_zero((float*)counters, 2 * (size_t)plan.S, s);
 }
 
-hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int step, int seg_begin,
-                                     hipStream_t s) {
+hipError_t launch_adjoint_batch_step(const AdjBatchPlan& plan, int launch, int step,
+                                     int seg_begin, hipStream_t s) {
   const int slot = step - seg_begin;
   if (slot < 0 || slot >= plan.S) return hipErrorInvalidValue;
   const AdjStreamArgs* dargs = (const AdjStreamArgs*)plan.meta;
   const int* dts = (const int*)(plan.meta + plan.args_bytes);
-  int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes);
+  const int* dls = (const int*)(plan.meta + plan.args_bytes + plan.ts_bytes);
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes + plan.ls_bytes);
   for (int c = 0; c < 2; ++c) {
     const int nc = plan.count[c];
     if (nc == 0) continue;
     const int grid = std::min(plan.total[c], plan.cus);
     const AdjStreamArgs* a = dargs + 2 * plan.base[c];
     const int* ts = dts + plan.base[c] + c;
+    const int* ls = dls + plan.base[c];
     int* ctr = counters + 2 * slot + c;
     if (c == 0)
       hipLaunchKernelGGL((k_adj_stream_batch<1, 4, true>), dim3(grid), dim3(64 * 5), 0, s, a, ts,
-                         nc, plan.total[c], ctr, step, seg_begin);
+                         ls, nc, plan.total[c], ctr, launch, step, seg_begin);
     else
       hipLaunchKernelGGL((k_adj_stream_batch<2, 8, true>), dim3(grid), dim3(64 * 9), 0, s, a, ts,
-                         nc, plan.total[c], ctr, step, seg_begin);
+                         ls, nc, plan.total[c], ctr, launch, step, seg_begin);
   }
   return hipGetLastError();
 }

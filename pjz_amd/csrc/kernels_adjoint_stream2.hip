@@ -20,6 +20,7 @@
 // (108 B, + psi cotangents on PML cells) against 192 B for two one-step
 // launches.
 #include <algorithm>
+#include <vector>
 
 #include "stream_common.h"
 
@@ -43,6 +44,7 @@ struct Adj2Args {
   int X, Y, Z, lo, hi, NP, N, P;
   float dt;
   int LX, nty, ntiles;
+  int nsteps;  // steps of the simulation (batched runs)
 };
 
 template <int CPL>
@@ -372,12 +374,10 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
   for (int k = 0; k < 3; ++k) s.sigm[k] = cur.sig[k];
 }
 
+// One tile (WY rows x LX planes) of one pair of reversed steps.
 template <int CPL, int WY>
-__global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2(Adj2Args a) {
-  __shared__ Adj2Lds<CPL, WY> L;
+__device__ __forceinline__ void adj2_tile(const Adj2Args& a, int tile, Adj2Lds<CPL, WY>& L) {
   constexpr int NW = WY + 3;
-  int tile = blockIdx.x;
-  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
   const int Z = a.Z;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -441,15 +441,69 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2(Adj2Args a) {
   }
 }
 
+template <int CPL, int WY>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2(Adj2Args a) {
+  __shared__ Adj2Lds<CPL, WY> L;
+  // Tiles adjacent in y (sharing ghost rows) go to the same XCD.
+  int tile = blockIdx.x;
+  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
+  adj2_tile<CPL, WY>(a, tile, L);
+}
+
+// Batched pairs of reversed steps: persistent workgroups take (simulation,
+// tile) items from a device counter.  args holds two views per simulation
+// (which adjoint set is read); a simulation that joined the backward at
+// launch lstart[s] reads set (launch - lstart[s]) & 1.  D^step sits at slot
+// step - seg_begin of the view's D base, D^(step-1) one slot lower.
+template <int CPL, int WY>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2_batch(
+    const Adj2Args* __restrict__ args, const int* __restrict__ tile_start,
+    const int* __restrict__ lstart, int nsims, int total, int* __restrict__ counter, int launch,
+    int step, int seg_begin) {
+  __shared__ Adj2Lds<CPL, WY> L;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (t >= total) break;
+    int lo = 0, hi = nsims - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    if (step >= args[2 * lo].nsteps) continue;
+    Adj2Args a = args[2 * lo + ((launch - lstart[lo]) & 1)];
+    const long long f3 = 3LL * a.N;
+    a.D1 += (long long)(step - seg_begin) * f3;
+    a.D2 = a.D1 - f3;
+    adj2_tile<CPL, WY>(a, t - tile_start[lo], L);
+  }
+}
+
 // Whether the two-step reverse kernel handles this problem.
 bool adjoint_stream2_supported(const Geom& g) {
   return (g.Z <= 64) || (g.Z <= 128 && g.Z % 2 == 0);
 }
 
-hipError_t launch_adjoint_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
-                                  char* ws, const Layout& L, const float* D1, const float* D2,
-                                  int acur, hipStream_t s) {
-  if (!adjoint_stream2_supported(g)) return hipErrorInvalidValue;
+// 4 rows per tile (7 waves): the two levels' carried state needs ~200
+// registers a lane, which 2 waves per SIMD allow without spills (8 rows,
+// 11 waves, would cap it at 168 and spill: measured slower).
+static constexpr int kAdj2WY = 4;
+
+static int device_cus2() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return cus;
+}
+
+// Arguments of a pair of reversed steps reading adjoint set acur; LX = 0
+// picks x chunks giving about one workgroup per CU.
+static Adj2Args adj2_args(const Geom& g, const pjz_fdtd_problem& p, const float* mask, char* ws,
+                          const Layout& L, const float* D1, const float* D2, int acur, int LX) {
   Adj2Args a;
   const int in = acur ? 1 : 0, out = 1 - in;
   a.lEin = (const float*)(ws + L.lE_[in]);
@@ -475,21 +529,114 @@ hipError_t launch_adjoint_stream2(const Geom& g, const pjz_fdtd_problem& p, cons
   a.N = (int)g.N;
   a.P = g.X * g.Y * g.NP;
   a.dt = p.dt;
-  // 4 rows per tile (7 waves): the two levels' carried state needs ~200
-  // registers a lane, which 2 waves per SIMD allow without spills (8 rows,
-  // 11 waves, would cap it at 168 and spill: measured slower).
-  constexpr int WY = 4;
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess)
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  a.nty = (g.Y + WY - 1) / WY;
-  const int ntx = std::max(1, (cus + a.nty - 1) / a.nty);
-  a.LX = std::max(8, (g.X + ntx - 1) / ntx);
+  a.nty = (g.Y + kAdj2WY - 1) / kAdj2WY;
+  if (LX <= 0) {
+    const int ntx = std::max(1, (device_cus2() + a.nty - 1) / a.nty);
+    LX = std::max(8, (g.X + ntx - 1) / ntx);
+  }
+  a.LX = LX;
   a.ntiles = a.nty * ((g.X + a.LX - 1) / a.LX);
+  a.nsteps = steps_of(p);
+  return a;
+}
+
+hipError_t launch_adjoint_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                                  char* ws, const Layout& L, const float* D1, const float* D2,
+                                  int acur, hipStream_t s) {
+  if (!adjoint_stream2_supported(g)) return hipErrorInvalidValue;
+  Adj2Args a = adj2_args(g, p, mask, ws, L, D1, D2, acur, 0);
+  constexpr int WY = kAdj2WY;
   if (g.Z <= 64)
     hipLaunchKernelGGL((k_adj_stream2<1, WY>), dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
   else
     hipLaunchKernelGGL((k_adj_stream2<2, WY>), dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- batches
+// Meta layout: two argument views per simulation (cells-per-lane class
+// order) | tile prefix sums (n_c + 1 per class) | lstart (class order) |
+// queue counters (2 per launch slot of a segment).
+static constexpr int kAdj2BatchLX = 128;
+
+size_t adjoint2_batch_meta_bytes(int n, int S) {
+  return align256(sizeof(Adj2Args) * 2 * (size_t)n) + align256(sizeof(int) * (size_t)(n + 2)) +
+         align256(sizeof(int) * (size_t)n) + align256(sizeof(int) * 2 * (size_t)std::max(S, 1));
+}
+
+hipError_t adjoint2_batch_prepare(const BatchItem* items, int n, int S, const int* lstart,
+                                  char* meta, Adj2BatchPlan& plan, hipStream_t s) {
+  std::vector<int> cls[2];
+  for (int i = 0; i < n; ++i) {
+    if (!adjoint_stream2_supported(items[i].g)) return hipErrorInvalidValue;
+    cls[items[i].g.Z <= 64 ? 0 : 1].push_back(i);
+  }
+  plan.n = n;
+  plan.S = std::max(S, 1);
+  plan.meta = meta;
+  plan.cus = device_cus2();
+  plan.args_bytes = align256(sizeof(Adj2Args) * 2 * (size_t)n);
+  plan.ts_bytes = align256(sizeof(int) * (size_t)(n + 2));
+  plan.ls_bytes = align256(sizeof(int) * (size_t)n);
+  std::vector<char> host(plan.args_bytes + plan.ts_bytes + plan.ls_bytes, 0);
+  Adj2Args* hargs = (Adj2Args*)host.data();
+  int* hts = (int*)(host.data() + plan.args_bytes);
+  int* hls = (int*)(host.data() + plan.args_bytes + plan.ts_bytes);
+  int off = 0;
+  for (int c = 0; c < 2; ++c) {
+    plan.base[c] = off;
+    plan.count[c] = (int)cls[c].size();
+    int t = 0;
+    for (size_t k = 0; k < cls[c].size(); ++k) {
+      const int i = cls[c][k];
+      const BatchItem& it = items[i];
+      const int LX = std::min(kAdj2BatchLX, it.g.X);
+      for (int v = 0; v < 2; ++v)
+        hargs[2 * (off + k) + v] =
+            adj2_args(it.g, *it.p, it.mask, it.ws, *it.L, it.capture, it.capture, v, LX);
+      hts[off + c + k] = t;
+      hls[off + k] = lstart[i];
+      t += hargs[2 * (off + k)].ntiles;
+    }
+    hts[off + c + cls[c].size()] = t;
+    plan.total[c] = t;
+    off += (int)cls[c].size();
+  }
+  hipError_t e = hipMemcpyAsync(meta, host.data(), host.size(), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  // The host image must outlive the copy.
+  return hipStreamSynchronize(s);
+}
+
+hipError_t adjoint2_batch_begin_segment(const Adj2BatchPlan& plan, hipStream_t s) {
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes + plan.ls_bytes);
+  return launch_zero((float*)counters, 2 * (size_t)plan.S, s);
+}
+
+hipError_t launch_adjoint2_batch_step(const Adj2BatchPlan& plan, int launch, int step,
+                                      int seg_begin, hipStream_t s) {
+  const int slot = step - seg_begin;
+  if (slot < 1 || slot >= plan.S) return hipErrorInvalidValue;
+  const Adj2Args* dargs = (const Adj2Args*)plan.meta;
+  const int* dts = (const int*)(plan.meta + plan.args_bytes);
+  const int* dls = (const int*)(plan.meta + plan.args_bytes + plan.ts_bytes);
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes + plan.ls_bytes);
+  constexpr int WY = kAdj2WY;
+  for (int c = 0; c < 2; ++c) {
+    const int nc = plan.count[c];
+    if (nc == 0) continue;
+    const int grid = std::min(plan.total[c], plan.cus);
+    const Adj2Args* a = dargs + 2 * plan.base[c];
+    const int* ts = dts + plan.base[c] + c;
+    const int* ls = dls + plan.base[c];
+    int* ctr = counters + 2 * slot + c;
+    if (c == 0)
+      hipLaunchKernelGGL((k_adj_stream2_batch<1, WY>), dim3(grid), dim3(64 * (WY + 3)), 0, s, a,
+                         ts, ls, nc, plan.total[c], ctr, launch, step, seg_begin);
+    else
+      hipLaunchKernelGGL((k_adj_stream2_batch<2, WY>), dim3(grid), dim3(64 * (WY + 3)), 0, s, a,
+                         ts, ls, nc, plan.total[c], ctr, launch, step, seg_begin);
+  }
   return hipGetLastError();
 }
 
