@@ -1,13 +1,19 @@
 """MI355X-native fdtdz() Yee FDTD kernel behind pjz's call surface.
 
 ``fdtdz`` is the drop-in for the kernel pjz calls at
-``src/pjz/_field.py:254-269``; ``boundaries`` holds the helpers that build its
-inputs and project its snapshots (``src/pjz/_field.py:56-150``).
+``src/pjz/_field.py:254-269`` (``fdtdz_batch`` runs many at once);
+``boundaries`` holds the helpers that build its inputs and project its
+snapshots (``src/pjz/_field.py:56-150``); ``field``, ``scatter`` and
+``SimParams`` mirror pjz's ``field()`` / ``scatter()`` layer over it
+(``src/pjz/_field.py:12-53, 171-442``).
 """
 
 from .boundaries import (absorption_mask, frequency_components, output_phases,
                          pml_sigma, ramped_sin, sampling_interval, source_amplitude)
 from .fdtdz_torch import fdtdz, fdtdz_batch
+from .field import SimParams, field
+from .scatter import scatter
 
-__all__ = ["fdtdz", "fdtdz_batch", "absorption_mask", "pml_sigma", "ramped_sin", "sampling_interval",
-           "output_phases", "frequency_components", "source_amplitude"]
+__all__ = ["fdtdz", "fdtdz_batch", "field", "scatter", "SimParams", "absorption_mask", "pml_sigma",
+           "ramped_sin", "sampling_interval", "output_phases", "frequency_components",
+           "source_amplitude"]
