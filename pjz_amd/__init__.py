@@ -11,8 +11,8 @@ snapshots (``src/pjz/_field.py:56-150``); ``field``, ``scatter`` and
 from .boundaries import (absorption_mask, frequency_components, output_phases,
                          pml_sigma, ramped_sin, sampling_interval, source_amplitude)
 from .fdtdz_torch import fdtdz, fdtdz_batch
-from .field import SimParams, field
-from .scatter import scatter
+from ._field import SimParams, field
+from ._scatter import scatter
 
 __all__ = ["fdtdz", "fdtdz_batch", "field", "scatter", "SimParams", "absorption_mask", "pml_sigma",
            "ramped_sin", "sampling_interval", "output_phases", "frequency_components",

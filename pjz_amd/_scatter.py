@@ -13,7 +13,7 @@ from typing import Any, Sequence
 
 import torch
 
-from .field import SimParams, field, prop_axis, transverse_slice
+from ._field import SimParams, field, prop_axis, transverse_slice
 
 
 def amplitudes(beta, vals, x):

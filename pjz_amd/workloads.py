@@ -186,7 +186,7 @@ def overlap_objective(fields, mode, omegas, output_steps, dt=0.5, monitor_x=220)
   frequencies is field.project (pjz's pinv of the output phases).
   """
   import torch
-  from .field import project
+  from ._field import project
   ef = project(fields, omegas, output_steps, dt)  # (ww, 3, xx, yy, zz) complex
   plane = ef[:, 1:, monitor_x]                    # (ww, 2, yy, zz): Ey, Ez
   m = torch.as_tensor(mode, device=fields.device).to(plane.dtype)

@@ -5,7 +5,7 @@ import pytest
 import torch
 
 from oracle import field_ref
-from pjz_amd import field as Fd
+from pjz_amd import _field as Fd
 
 P = Fd.SimParams(omega_range=(2 * np.pi / 22, 2 * np.pi / 18), tt=700, absorption_padding=6,
                  absorption_coeff=4e-3, pml_widths=(6, 6), pml_alpha_coeff=0.05,

@@ -5,8 +5,8 @@ import pytest
 import torch
 
 from oracle import scatter_ref
-from pjz_amd import field as Fd
-from pjz_amd import scatter as Sc
+from pjz_amd import _field as Fd
+from pjz_amd import _scatter as Sc
 
 P = Fd.SimParams(omega_range=(2 * np.pi / 21, 2 * np.pi / 19), tt=600, absorption_padding=6,
                  absorption_coeff=4e-3, pml_widths=(6, 6), pml_alpha_coeff=0.05,
