@@ -12,8 +12,10 @@ from .boundaries import (absorption_mask, frequency_components, output_phases,
                          pml_sigma, ramped_sin, sampling_interval, source_amplitude)
 from .fdtdz_torch import fdtdz, fdtdz_batch
 from ._field import SimParams, field
+from ._mode import mode
 from ._scatter import scatter
 
-__all__ = ["fdtdz", "fdtdz_batch", "field", "scatter", "SimParams", "absorption_mask", "pml_sigma",
+__all__ = ["fdtdz", "fdtdz_batch", "field", "scatter", "mode", "SimParams", "absorption_mask",
+           "pml_sigma",
            "ramped_sin", "sampling_interval", "output_phases", "frequency_components",
            "source_amplitude"]
