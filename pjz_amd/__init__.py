@@ -5,17 +5,20 @@
 ``boundaries`` holds the helpers that build its inputs and project its
 snapshots (``src/pjz/_field.py:56-150``); ``field``, ``scatter`` and
 ``SimParams`` mirror pjz's ``field()`` / ``scatter()`` layer over it
-(``src/pjz/_field.py:12-53, 171-442``).
+(``src/pjz/_field.py:12-53, 171-442``); ``mode``, ``epsilon`` / ``render``
+and ``density`` are pjz's set-up steps before it (``src/pjz/_mode.py``,
+``_epsilon.py``, ``_density.py``), run on the tensors' device.
 """
 
 from .boundaries import (absorption_mask, frequency_components, output_phases,
                          pml_sigma, ramped_sin, sampling_interval, source_amplitude)
 from .fdtdz_torch import fdtdz, fdtdz_batch
+from ._density import density
+from ._epsilon import epsilon, render
 from ._field import SimParams, field
 from ._mode import mode
 from ._scatter import scatter
 
-__all__ = ["fdtdz", "fdtdz_batch", "field", "scatter", "mode", "SimParams", "absorption_mask",
-           "pml_sigma",
-           "ramped_sin", "sampling_interval", "output_phases", "frequency_components",
-           "source_amplitude"]
+__all__ = ["fdtdz", "fdtdz_batch", "field", "scatter", "mode", "epsilon", "render", "density",
+           "SimParams", "absorption_mask", "pml_sigma", "ramped_sin", "sampling_interval",
+           "output_phases", "frequency_components", "source_amplitude"]

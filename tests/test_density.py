@@ -1,0 +1,128 @@
+"""Three-field density (pjz_amd.density), torch port of pjz's
+``src/pjz/_density.py``.
+
+The reference has no tests for it, so each stage is pinned against something
+independent of the code under test: the cone's closed forms (radius 1 is the
+identity; radius 1.5 has centre/edge/corner weights 1.5, 0.5, 1.5 - sqrt 2
+before normalisation), the blur against scipy.signal.convolve on an
+edge-padded array, the boundary mask / projection / inflection / loss against
+per-pixel loops written from their definitions, and invariants (a constant
+variable is a fixed point of the blur; a binary variable far from the
+thresholds has zero loss).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.signal
+import torch
+
+from pjz_amd import _density as D
+
+
+def test_cone_closed_forms():
+  np.testing.assert_allclose(D.cone(1.0, torch.float64).numpy(),
+                             [[0, 0, 0], [0, 1, 0], [0, 0, 0]])
+  corner = 1.5 - math.sqrt(2)
+  w = np.array([[corner, 0.5, corner], [0.5, 1.5, 0.5], [corner, 0.5, corner]])
+  np.testing.assert_allclose(D.cone(1.5, torch.float64).numpy(), w / w.sum(), rtol=1e-12)
+  for r in (2.0, 3.7, 6.5):
+    k = D.cone(r, torch.float64).numpy()
+    assert k.shape == (2 * math.ceil(r - 0.5) + 1,) * 2
+    assert k.sum() == pytest.approx(1.0) and np.allclose(k, k.T) and np.allclose(k, k[::-1])
+
+
+@pytest.mark.parametrize("radius", [1.0, 2.5, 4.0])
+def test_blur_matches_scipy(radius):
+  rng = np.random.default_rng(1)
+  u = rng.random((17, 23))
+  k = D.cone(radius, torch.float64).numpy()
+  r = k.shape[0] // 2
+  ref = scipy.signal.convolve(np.pad(u, r, mode="edge"), k, mode="valid")
+  got = D.blur(torch.tensor(u), radius).numpy()
+  np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14)
+  np.testing.assert_allclose(D.blur(torch.full((9, 7), 0.3, dtype=torch.float64), radius).numpy(),
+                             0.3, rtol=1e-12)
+
+
+def loop_boundary(v):
+  xx, yy = v.shape
+  b = v >= 0
+  out = np.zeros_like(b)
+  for i in range(xx):
+    for j in range(yy):
+      for di, dj in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+        ii, jj = min(max(i + di, 0), xx - 1), min(max(j + dj, 0), yy - 1)
+        out[i, j] |= b[ii, jj] != b[i, j]
+  return out
+
+
+def loop_inflection(u, c):
+  xx, yy = u.shape
+  out = np.zeros_like(u)
+  for i in range(xx):
+    for j in range(yy):
+      gx = u[min(i + 1, xx - 1), j] - u[max(i - 1, 0), j]
+      gy = u[i, min(j + 1, yy - 1)] - u[i, max(j - 1, 0)]
+      out[i, j] = np.exp(-c * (gx * gx + gy * gy))
+  return out
+
+
+def test_projection_and_loss_against_loops():
+  rng = np.random.default_rng(2)
+  u = rng.random((12, 15))
+  uf = D.blur(torch.tensor(u), 2.0).numpy()
+  bnd = loop_boundary(uf - 0.5)
+  np.testing.assert_array_equal(D.boundary_cells(torch.tensor(uf - 0.5)).numpy(), bnd)
+  proj = np.where(bnd, uf, (uf > 0.5).astype(float) + 0.5 * (uf == 0.5))
+  np.testing.assert_allclose(D.project(torch.tensor(uf), 0.5).numpy(), proj, rtol=1e-12)
+  infl = loop_inflection(uf, 1.7)
+  np.testing.assert_allclose(D.inflection(torch.tensor(uf), 1.7).numpy(), infl, rtol=1e-12)
+  loss = np.zeros_like(uf)
+  for i in range(uf.shape[0]):
+    for j in range(uf.shape[1]):
+      lo, hi = min(0.0, uf[i, j] - 0.75), min(0.0, 0.25 - uf[i, j])
+      loss[i, j] = infl[i, j] * (proj[i, j] * lo * lo + (1 - proj[i, j]) * hi * hi)
+  np.testing.assert_allclose(
+      D.geom_loss(torch.tensor(proj), torch.tensor(uf), 1.7, 0.25, 0.75).numpy(), loss,
+      rtol=1e-12, atol=1e-300)
+  dens, l2 = D.density(torch.tensor(u), 2.0, 0.3, c=1.7)
+  np.testing.assert_allclose(dens.numpy(), 0.3 * proj + 0.7 * uf, rtol=1e-12)
+  np.testing.assert_allclose(l2.numpy(), loss, rtol=1e-12, atol=1e-300)
+
+
+def test_binary_far_from_thresholds():
+  """A large solid square: alpha = 1 leaves only the blurred rim non-binary,
+  and pixels well inside either phase carry no loss."""
+  u = np.zeros((40, 40))
+  u[10:30, 10:30] = 1
+  dens, loss = D.density(torch.tensor(u), 3.0, 1.0)
+  dens, loss = dens.numpy(), loss.numpy()
+  assert dens[20, 20] == 1 and dens[2, 2] == 0 and loss[20, 20] == 0 and loss[2, 2] == 0
+  assert set(np.unique(dens[~D.boundary_cells(D.blur(torch.tensor(u), 3.0) - 0.5).numpy()])) \
+      <= {0.0, 1.0}
+
+
+def test_differentiable():
+  u = torch.rand((16, 16), dtype=torch.float64, requires_grad=True)
+  dens, loss = D.density(u, 2.5, 0.5)
+  (dens.sum() + loss.sum()).backward()
+  assert u.grad.shape == (16, 16) and torch.isfinite(u.grad).all()
+
+
+@pytest.mark.gpu
+def test_density_on_the_gpu(gpu):
+  rng = np.random.default_rng(4)
+  u = rng.random((64, 48))
+  d0, l0 = D.density(torch.tensor(u), 3.5, 0.6)
+  d1, l1 = D.density(torch.tensor(u, dtype=torch.float32, device=gpu), 3.5, 0.6)
+  assert d1.device.type == "cuda"
+  # Pixels whose blurred value, or a neighbour's, sits within float32
+  # rounding of eta can binarise differently; compare everything else.
+  uf = D.blur(torch.tensor(u), 3.5).numpy()
+  near = np.abs(uf - 0.5) <= 1e-5
+  ok = ~(near | np.roll(near, 1, 0) | np.roll(near, -1, 0) | np.roll(near, 1, 1) |
+         np.roll(near, -1, 1))
+  np.testing.assert_allclose(d1.cpu().numpy()[ok], d0.numpy()[ok], rtol=1e-5, atol=1e-6)
+  np.testing.assert_allclose(l1.cpu().numpy()[ok], l0.numpy()[ok], rtol=1e-4, atol=1e-7)
