@@ -214,9 +214,11 @@ __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y
   for (int k = 0; k < 3; ++k) load_col<CPL, vec>(P.e[k], a.Ein + (k * N + o1), zb, Z, a1);
 #pragma unroll
   for (int k = 0; k < 3; ++k) load_col<CPL, vec>(P.h[k], a.Ein + (a.hoff + k * N + o), zb, Z, active);
+  // cb is read by this row's tile and the ghost rows of its y neighbours,
+  // which run at the same time; a nontemporal load measured slightly faster.
 #pragma unroll
   for (int k = 0; k < 3; ++k)
-    load_col<CPL, vec>(P.cb[k], a.cb + (k * N + o), zb, Z, active && e_rows);
+    load_col<CPL, vec, true>(P.cb[k], a.cb + (k * N + o), zb, Z, active && e_rows);
   if (top) {
     const bool v = active && yc + 1 < a.Y;
     load_col<CPL, vec>(P.up[0], a.Ein + (o + Z), zb, Z, v);

@@ -6,8 +6,12 @@
 
 namespace pjz {
 
-// Loads CPL consecutive z cells of a row; cells at z >= Z read as zero.
-template <int CPL, bool vec>
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Loads CPL consecutive z cells of a row; cells at z >= Z read as zero.  NT
+// marks the load nontemporal (streaming: the line is not kept for reuse).
+template <int CPL, bool vec, bool NT = false>
 __device__ inline void load_col(float (&v)[CPL], const float* __restrict__ p, int zb, int Z,
                                 bool valid) {
   if (!valid || zb >= Z) {
@@ -17,13 +21,15 @@ __device__ inline void load_col(float (&v)[CPL], const float* __restrict__ p, in
   }
   if (vec) {
     if constexpr (CPL == 4) {
-      const float4 t = *reinterpret_cast<const float4*>(p + zb);
+      const f32x4* q = reinterpret_cast<const f32x4*>(p + zb);
+      const f32x4 t = NT ? __builtin_nontemporal_load(q) : *q;
       v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
     } else if constexpr (CPL == 2) {
-      float2 t = *reinterpret_cast<const float2*>(p + zb);
+      const f32x2* q = reinterpret_cast<const f32x2*>(p + zb);
+      const f32x2 t = NT ? __builtin_nontemporal_load(q) : *q;
       v[0] = t.x; v[1] = t.y;
     } else {
-      v[0] = p[zb];
+      v[0] = NT ? __builtin_nontemporal_load(p + zb) : p[zb];
     }
   } else {
 #pragma unroll
@@ -31,16 +37,22 @@ __device__ inline void load_col(float (&v)[CPL], const float* __restrict__ p, in
   }
 }
 
+// Stores CPL consecutive z cells (z < Z only).  Field results are written
+// nontemporally: no kernel re-reads them before the next launch, and keeping
+// them out of the cache leaves room for the inputs that neighbouring tiles
+// share (ghost rows).
 template <int CPL, bool vec>
 __device__ inline void store_col(float* __restrict__ p, const float (&v)[CPL], int zb, int Z) {
   if (zb >= Z) return;
   if (vec) {
     if constexpr (CPL == 4) {
-      *reinterpret_cast<float4*>(p + zb) = make_float4(v[0], v[1], v[2], v[3]);
+      f32x4 t = {v[0], v[1], v[2], v[3]};
+      __builtin_nontemporal_store(t, reinterpret_cast<f32x4*>(p + zb));
     } else if constexpr (CPL == 2) {
-      *reinterpret_cast<float2*>(p + zb) = make_float2(v[0], v[1]);
+      f32x2 t = {v[0], v[1]};
+      __builtin_nontemporal_store(t, reinterpret_cast<f32x2*>(p + zb));
     } else {
-      p[zb] = v[0];
+      __builtin_nontemporal_store(v[0], p + zb);
     }
   } else {
 #pragma unroll
@@ -58,6 +70,9 @@ using rsrc_t = __amdgpu_buffer_rsrc_t;
 constexpr unsigned kOff = 0x80000000u;
 // Resource word 3 of a raw (untyped, unswizzled) buffer on gfx9-family GPUs.
 constexpr int kRsrcWord3 = 0x00020000;
+// Cache-policy operand of buffer stores: nontemporal (the nt bit on gfx950),
+// for the same reason as store_col's.
+constexpr int kStoreNT = 2;
 
 __device__ inline rsrc_t make_rsrc(const void* base, unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
@@ -72,7 +87,7 @@ __device__ inline float bload1(rsrc_t r, unsigned voff, unsigned soff) {
 }
 
 __device__ inline void bstore1(float v, rsrc_t r, unsigned voff, unsigned soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, kStoreNT);
 }
 
 // CPL consecutive floats at byte offset voff + soff.
@@ -101,12 +116,12 @@ __device__ inline void bstore(const float (&v)[CPL], rsrc_t r, unsigned voff, un
     t.y = __builtin_bit_cast(unsigned, v[1]);
     t.z = __builtin_bit_cast(unsigned, v[2]);
     t.w = __builtin_bit_cast(unsigned, v[3]);
-    __builtin_amdgcn_raw_buffer_store_b128(t, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(t, r, voff, soff, kStoreNT);
   } else if constexpr (CPL == 2) {
     u32x2 t;
     t.x = __builtin_bit_cast(unsigned, v[0]);
     t.y = __builtin_bit_cast(unsigned, v[1]);
-    __builtin_amdgcn_raw_buffer_store_b64(t, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, voff, soff, kStoreNT);
   } else {
     bstore1(v[0], r, voff, soff);
   }
