@@ -73,6 +73,32 @@ def cpu_baseline(kw, target_s=10.0):
                     f"(set-up of {t_setup:.1f} s excluded)"}
 
 
+def copy_ceiling_gbs(dev, gib=1, reps=10):
+  """Device-copy bandwidth (GB/s, read + write) of the library's nontemporal
+  copy kernel on `gib` GiB, HIP events on the launching stream."""
+  import ctypes
+  import torch
+  from pjz_amd import _lib
+  lib = _lib.load()
+  n = gib << 30
+  src = torch.ones(n // 4, dtype=torch.float32, device=dev)
+  dst = torch.empty_like(src)
+  stream = torch.cuda.current_stream(dev)
+  call = lambda: _lib.check(lib.pjz_fdtd_copy(ctypes.c_void_p(src.data_ptr()),
+                                              ctypes.c_void_p(dst.data_ptr()), n,
+                                              ctypes.c_void_p(stream.cuda_stream)), "copy")
+  call(), call()
+  start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  start.record(stream)
+  for _ in range(reps):
+    call()
+  end.record(stream)
+  torch.cuda.synchronize()
+  if not torch.equal(src[:1024], dst[:1024]):
+    raise RuntimeError("copy kernel produced wrong data")
+  return 2.0 * n * reps / (start.elapsed_time(end) * 1e-3) / 1e9
+
+
 def measured_traffic(kernel, workload):
   """HBM bytes per launch from the committed rocprofv3 counter summary."""
   path = os.path.join(ROOT, "profiles", f"k_{kernel}_{workload}.json")
@@ -322,6 +348,13 @@ def main():
                    "algorithmic_bytes_per_cell": round(bpc, 4),
                    "avg_launch_us": round(avg_launch_s * 1e6, 2)},
   }
+  # The practical ceiling: a nontemporal 1 GiB device copy on the same GPU,
+  # and the kernel's measured HBM traffic rate as a fraction of it.
+  ceiling = copy_ceiling_gbs(dev)
+  line["roofline"]["copy_ceiling_gbs"] = round(ceiling, 1)
+  traffic = line["roofline"]["traffic"]
+  if traffic:
+    line["roofline"]["traffic_frac_of_copy"] = round(traffic / avg_launch_s / 1e9 / ceiling, 4)
   if not args.no_cpu_baseline:
     np_kw = builder(steps=args.time_steps) if args.time_steps else builder()
     line["cpu_baseline"] = cpu_baseline(np_kw)

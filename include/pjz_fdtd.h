@@ -164,6 +164,15 @@ PJZ_FDTD_API int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsil
                       const float* grad_out, float* grad_eps, void* workspace,
                       size_t workspace_bytes, void* stream);
 
+/*
+ * Diagnostic: copies `bytes` (a multiple of 16; both pointers 16-byte
+ * aligned) from src to dst on the device with nontemporal loads and stores,
+ * stream-ordered.  No counterpart in pjz: bench.py times it on 1 GiB to
+ * report the device-copy bandwidth the steppers' measured HBM traffic is
+ * compared against.
+ */
+PJZ_FDTD_API int pjz_fdtd_copy(const void* src, void* dst, size_t bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@ EXPORTS = (
     "pjz_fdtd_batch_workspace_bytes",
     "pjz_fdtd_batch_forward",
     "pjz_fdtd_batch_backward",
+    "pjz_fdtd_copy",
 )
 
 
@@ -86,6 +87,8 @@ def load():
     lib.pjz_fdtd_batch_backward.restype = ctypes.c_int
     lib.pjz_fdtd_batch_backward.argtypes = ([P, ctypes.c_int] + [ctypes.POINTER(vp)] * 9 +
                                             [vp, sz, vp])
+    lib.pjz_fdtd_copy.restype = ctypes.c_int
+    lib.pjz_fdtd_copy.argtypes = [vp, vp, sz, vp]
     if lib.pjz_fdtd_abi_version() != ABI_VERSION:
       raise RuntimeError("libpjzfdtd.so ABI version mismatch; rebuild it")
     _lib = lib

@@ -4,6 +4,7 @@
 #include <string.h>
 #include <stdarg.h>
 #include <limits.h>
+#include <stdint.h>
 
 #include <algorithm>
 #include <map>
@@ -286,6 +287,15 @@ extern "C" {
 int pjz_fdtd_abi_version(void) { return PJZ_FDTD_ABI_VERSION; }
 
 const char* pjz_fdtd_last_error(void) { return g_err; }
+
+int pjz_fdtd_copy(const void* src, void* dst, size_t bytes, void* stream) {
+  if (!src || !dst) return fail(PJZ_FDTD_EINVAL, "copy: NULL pointer");
+  if (bytes % 16 || (uintptr_t)src % 16 || (uintptr_t)dst % 16)
+    return fail(PJZ_FDTD_EINVAL, "copy: bytes and both pointers must be multiples of 16");
+  if (bytes == 0) return PJZ_FDTD_OK;
+  hipError_t e = pjz::launch_copy_stream(dst, src, bytes / 16, (hipStream_t)stream);
+  return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "copy");
+}
 
 int pjz_fdtd_num_outputs(const pjz_fdtd_problem* p, int32_t* nn) {
   int rc = check(p);

@@ -134,6 +134,9 @@ hipError_t launch_zero(float* ptr, size_t n, hipStream_t s);
 // Copies n floats with a copy kernel.
 hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t s);
 
+// Copies n16 16-byte words with nontemporal loads and stores (bandwidth probe).
+hipError_t launch_copy_stream(void* dst, const void* src, size_t n16, hipStream_t s);
+
 hipError_t run_reference(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                          const Src& src, const OutWin& win, char* ws, const Layout& L,
                          const StepRange& r, hipStream_t s);

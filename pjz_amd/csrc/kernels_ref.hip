@@ -69,6 +69,24 @@ hipError_t launch_prepare(const Geom& g, const pjz_fdtd_problem& p, const float*
   return hipGetLastError();
 }
 
+// Streaming copy of n16 16-byte words with nontemporal loads and stores: the
+// device-copy ceiling the steppers' HBM traffic is compared against.
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(1024) void k_copy_stream(const f32x4v* __restrict__ src,
+                                                      f32x4v* __restrict__ dst, long long n16) {
+  long long i = blockIdx.x * 1024LL + threadIdx.x;
+  const long long stride = (long long)gridDim.x * 1024;
+  for (; i < n16; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+hipError_t launch_copy_stream(void* dst, const void* src, size_t n16, hipStream_t s) {
+  unsigned nb = blocks_for((long long)n16, 1024);
+  if (nb > 16384u) nb = 16384u;
+  hipLaunchKernelGGL(k_copy_stream, dim3(nb), dim3(1024), 0, s, (const f32x4v*)src, (f32x4v*)dst,
+                     (long long)n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(float* ptr, size_t n, hipStream_t s) {
   unsigned nb = blocks_for((long long)n, 256);
   if (nb > 65536u) nb = 65536u;

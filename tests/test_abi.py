@@ -85,3 +85,17 @@ def test_bad_shapes_raise_value_error():
 def test_fdtdz_fails_loudly_without_gpu():
   with pytest.raises(RuntimeError, match="GPU"):
     F.fdtdz(**workloads.layered_64(steps=20))
+
+
+def test_copy_rejects_bad_arguments_before_any_launch():
+  lib = _lib.load()
+  buf = ctypes.create_string_buffer(64)
+  base = ctypes.addressof(buf)
+  assert lib.pjz_fdtd_copy(None, ctypes.c_void_p(base), 16, None) == _lib.EINVAL
+  assert lib.pjz_fdtd_copy(ctypes.c_void_p(base), ctypes.c_void_p(base), 15, None) == _lib.EINVAL
+  misaligned = base + (16 - base % 16) + 4
+  assert lib.pjz_fdtd_copy(ctypes.c_void_p(misaligned), ctypes.c_void_p(base), 16,
+                           None) == _lib.EINVAL
+  assert "16" in lib.pjz_fdtd_last_error().decode()
+  aligned = ctypes.c_void_p(base + (16 - base % 16) % 16)
+  assert lib.pjz_fdtd_copy(aligned, aligned, 0, None) == _lib.OK

@@ -181,3 +181,16 @@ def test_autotune_with_a_single_step():
   kw = random_problem(0, seed=31)
   kw.update(output_steps=(0, 1, 1))
   assert rel_l2(run_gpu(kw, "autotune"), C.fdtdz(**kw)) <= TOL
+
+
+def test_copy_probe_copies():
+  import ctypes
+  from pjz_amd import _lib
+  src = torch.arange(1 << 20, dtype=torch.float32, device="cuda")
+  dst = torch.zeros_like(src)
+  stream = torch.cuda.current_stream()
+  _lib.check(_lib.load().pjz_fdtd_copy(ctypes.c_void_p(src.data_ptr()),
+                                       ctypes.c_void_p(dst.data_ptr()), src.numel() * 4,
+                                       ctypes.c_void_p(stream.cuda_stream)), "copy")
+  torch.cuda.synchronize()
+  assert torch.equal(src, dst)
