@@ -165,6 +165,30 @@ PJZ_FDTD_API int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsil
                       size_t workspace_bytes, void* stream);
 
 /*
+ * Waveguide modes: the subspace iteration of pjz's mode()
+ * (src/pjz/_mode.py:99-142, _subspace_iteration, with the operator of
+ * _mode.py:22-50), run to convergence inside one stream-ordered launch.
+ *   epsilon  (3, xx, yy)          eps_x, eps_y, eps_z of the cross-section in
+ *                                 propagate-along-z form
+ *   omega    (ww,), shift (ww,)   angular frequency and eigenvalue shift
+ *   x        (ww, 2, xx, yy, mm)  start vectors in; on return the last
+ *                                 orthonormal basis (modes innermost)
+ *   w, err   (ww, mm)             Rayleigh quotients <x_k, A x_k> and residuals
+ *                                 |A x_k - w_k x_k|, in basis order (unsorted)
+ *   iters    (ww,)                iterations run per frequency
+ * Each frequency iterates until max_iters or until every residual is <= tol
+ * (pjz tests max(err) > tol before each iteration, over all frequencies at
+ * once).  1 <= mm <= 10.  Workspace from pjz_fdtd_mode_workspace_bytes.
+ */
+PJZ_FDTD_API int pjz_fdtd_mode_workspace_bytes(int32_t ww, int32_t xx, int32_t yy, int32_t mm,
+                                               size_t* bytes);
+PJZ_FDTD_API int pjz_fdtd_mode_solve(int32_t ww, int32_t xx, int32_t yy, int32_t mm,
+                                     const float* epsilon, const float* omega, const float* shift,
+                                     float* x, float* w, float* err, int32_t* iters,
+                                     int32_t max_iters, float tol, void* workspace,
+                                     size_t workspace_bytes, void* stream);
+
+/*
  * Diagnostic: copies `bytes` (a multiple of 16; both pointers 16-byte
  * aligned) from src to dst on the device with nontemporal loads and stores,
  * stream-ordered.  No counterpart in pjz: bench.py times it on 1 GiB to

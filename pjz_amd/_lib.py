@@ -27,6 +27,8 @@ EXPORTS = (
     "pjz_fdtd_batch_forward",
     "pjz_fdtd_batch_backward",
     "pjz_fdtd_copy",
+    "pjz_fdtd_mode_workspace_bytes",
+    "pjz_fdtd_mode_solve",
 )
 
 
@@ -89,6 +91,11 @@ def load():
                                             [vp, sz, vp])
     lib.pjz_fdtd_copy.restype = ctypes.c_int
     lib.pjz_fdtd_copy.argtypes = [vp, vp, sz, vp]
+    i32 = ctypes.c_int32
+    lib.pjz_fdtd_mode_workspace_bytes.restype = ctypes.c_int
+    lib.pjz_fdtd_mode_workspace_bytes.argtypes = [i32] * 4 + [ctypes.POINTER(sz)]
+    lib.pjz_fdtd_mode_solve.restype = ctypes.c_int
+    lib.pjz_fdtd_mode_solve.argtypes = [i32] * 4 + [vp] * 7 + [i32, ctypes.c_float, vp, sz, vp]
     if lib.pjz_fdtd_abi_version() != ABI_VERSION:
       raise RuntimeError("libpjzfdtd.so ABI version mismatch; rebuild it")
     _lib = lib

@@ -1,4 +1,4 @@
-"""Waveguide modes on the GPU: pjz's ``mode()`` solver in torch.
+"""Waveguide modes on the GPU: pjz's ``mode()`` solver.
 
 Mirrors ``src/pjz/_mode.py``: the waveguide operator of a 2-D cross-section
 (``_operator``, ``_mode.py:22-50``, periodic forward/backward differences),
@@ -6,10 +6,17 @@ subspace iteration with a QR re-orthogonalisation every step
 (``_subspace_iteration``, ``:99-142``), a shift equal to the largest
 eigenvalue from ``shift_iters`` iterations on one vector, wavevector
 ``sqrt(w + shift)``, Poynting normalisation (``_poynting``, ``:72-76``) and
-the axis / excitation conventions of ``mode()`` (``:145-257``).  Everything
-runs where ``epsilon`` lives (a CUDA tensor keeps the solve on the GPU); the
-QR is ``torch.linalg.qr``.  Results are float32 like pjz's.
+the axis / excitation conventions of ``mode()`` (``:145-257``).
+
+On a CUDA ``epsilon`` with at most 10 modes the iteration is one launch of
+the library's HIP kernel (``pjz_fdtd_mode_solve``: one workgroup per
+frequency runs the whole loop, CholeskyQR2 for the QR); a missing library
+raises.  CPU tensors, and more than 10 modes, run the same iteration as torch
+ops where ``epsilon`` lives (QR by ``torch.linalg.qr``).  Results are float32
+like pjz's.
 """
+
+import ctypes
 
 from typing import Optional
 
@@ -107,6 +114,40 @@ def _subspace_iteration(op, x, n, tol):
   return w, x, err, i
 
 
+def _subspace_iteration_hip(epsilon, omega, shift, x, n, tol):
+  """The iteration of ``_subspace_iteration`` for the operator
+  ``_operator(epsilon, omega, shift)``, run by the HIP kernel on the device
+  of ``x`` (ww, 2, xx, yy, mm); same results, sorted the same way."""
+  from . import _lib
+  lib = _lib.load()
+  ww, _, xx, yy, mm = x.shape
+  dev = x.device
+  f32 = dict(dtype=torch.float32, device=dev)
+  eps = torch.as_tensor(epsilon, **f32).contiguous()
+  om = torch.as_tensor(omega, **f32).reshape(ww).contiguous()
+  sh = torch.as_tensor(shift, **f32).reshape(ww).contiguous()
+  x = torch.as_tensor(x, **f32).contiguous().clone()
+  nbytes = ctypes.c_size_t()
+  _lib.check(lib.pjz_fdtd_mode_workspace_bytes(ww, xx, yy, mm, ctypes.byref(nbytes)), "mode")
+  ws = torch.empty(max(nbytes.value // 4, 1), **f32)
+  w = torch.empty((ww, mm), **f32)
+  err = torch.empty((ww, mm), **f32)
+  iters = torch.empty((ww,), dtype=torch.int32, device=dev)
+  stream = torch.cuda.current_stream(dev).cuda_stream
+  ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+  _lib.check(lib.pjz_fdtd_mode_solve(ww, xx, yy, mm, ptr(eps), ptr(om), ptr(sh), ptr(x), ptr(w),
+                                     ptr(err), ptr(iters), int(n), float(tol), ptr(ws),
+                                     nbytes.value, ctypes.c_void_p(stream)), "mode")
+  inds = torch.flip(torch.argsort(w, dim=-1), dims=(-1,))
+  x = torch.take_along_dim(x, inds[:, None, None, None, :], dim=-1)
+  w = torch.take_along_dim(w, inds, dim=-1)
+  err = torch.take_along_dim(err, inds, dim=-1)
+  return w, x, err, int(iters.max())
+
+
+HIP_MAX_MODES = 10
+
+
 def _random(shape, device, seed=0):
   gen = torch.Generator(device="cpu").manual_seed(seed)
   return torch.randn(shape, generator=gen, dtype=torch.float32).to(device)
@@ -157,10 +198,16 @@ def mode(epsilon, omega, num_modes: int, init: Optional[torch.Tensor] = None,
   epsilon = torch.squeeze(epsilon)
   init = torch.reshape(init, mode_shape)
 
-  shift, _, _, _ = _subspace_iteration(_operator(epsilon, omega, torch.zeros_like(omega)),
-                                       _random(mode_shape[:-1] + (1,), device), shift_iters, tol)
-  w, x, err, iters = _subspace_iteration(_operator(epsilon, omega, shift[:, 0]), init, max_iters,
-                                         tol)
+  if epsilon.is_cuda and num_modes <= HIP_MAX_MODES:
+    shift, _, _, _ = _subspace_iteration_hip(epsilon, omega, torch.zeros_like(omega),
+                                             _random(mode_shape[:-1] + (1,), device), shift_iters,
+                                             tol)
+    w, x, err, iters = _subspace_iteration_hip(epsilon, omega, shift[:, 0], init, max_iters, tol)
+  else:
+    shift, _, _, _ = _subspace_iteration(_operator(epsilon, omega, torch.zeros_like(omega)),
+                                         _random(mode_shape[:-1] + (1,), device), shift_iters, tol)
+    w, x, err, iters = _subspace_iteration(_operator(epsilon, omega, shift[:, 0]), init,
+                                           max_iters, tol)
   wavevector = torch.sqrt(w + shift)
   x = x / torch.sqrt(_poynting(wavevector, omega, epsilon, x))[:, None, None, None, :]
   exc = torch.flip(x, dims=(1,))  # field-to-excitation flip

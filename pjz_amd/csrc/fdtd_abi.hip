@@ -288,6 +288,43 @@ int pjz_fdtd_abi_version(void) { return PJZ_FDTD_ABI_VERSION; }
 
 const char* pjz_fdtd_last_error(void) { return g_err; }
 
+static int check_mode(int32_t ww, int32_t xx, int32_t yy, int32_t mm) {
+  if (ww < 1 || ww > 65535) return fail(PJZ_FDTD_EINVAL, "mode: ww = %d out of [1, 65535]", ww);
+  if (xx < 1 || yy < 1) return fail(PJZ_FDTD_EINVAL, "mode: cross-section %d x %d", xx, yy);
+  if (mm < 1 || mm > pjz::kModeMaxModes)
+    return fail(PJZ_FDTD_EINVAL, "mode: num_modes = %d out of [1, %d]", mm, pjz::kModeMaxModes);
+  if (2LL * xx * yy * mm > (long long)INT_MAX / 2)
+    return fail(PJZ_FDTD_EINVAL, "mode: 2 x %d x %d x %d values exceed the kernel's 32-bit indexing",
+                xx, yy, mm);
+  return PJZ_FDTD_OK;
+}
+
+int pjz_fdtd_mode_workspace_bytes(int32_t ww, int32_t xx, int32_t yy, int32_t mm, size_t* bytes) {
+  int rc = check_mode(ww, xx, yy, mm);
+  if (rc) return rc;
+  if (!bytes) return fail(PJZ_FDTD_EINVAL, "mode: bytes is NULL");
+  *bytes = pjz::mode_workspace_floats(ww, xx, yy, mm) * sizeof(float);
+  return PJZ_FDTD_OK;
+}
+
+int pjz_fdtd_mode_solve(int32_t ww, int32_t xx, int32_t yy, int32_t mm, const float* epsilon,
+                        const float* omega, const float* shift, float* x, float* w, float* err,
+                        int32_t* iters, int32_t max_iters, float tol, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  int rc = check_mode(ww, xx, yy, mm);
+  if (rc) return rc;
+  if (!epsilon || !omega || !shift || !x || !w || !err || !iters || !workspace)
+    return fail(PJZ_FDTD_EINVAL, "mode: NULL array");
+  if (max_iters < 0) return fail(PJZ_FDTD_EINVAL, "mode: max_iters = %d < 0", max_iters);
+  if (!(tol >= 0.0f)) return fail(PJZ_FDTD_EINVAL, "mode: tol must be >= 0");
+  const size_t need = pjz::mode_workspace_floats(ww, xx, yy, mm) * sizeof(float);
+  if (workspace_bytes < need)
+    return fail(PJZ_FDTD_ESPACE, "mode: workspace of %zu bytes, need %zu", workspace_bytes, need);
+  hipError_t e = pjz::launch_mode(ww, xx, yy, mm, epsilon, omega, shift, x, w, err, iters,
+                                  max_iters, tol, (float*)workspace, (hipStream_t)stream);
+  return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "mode");
+}
+
 int pjz_fdtd_copy(const void* src, void* dst, size_t bytes, void* stream) {
   if (!src || !dst) return fail(PJZ_FDTD_EINVAL, "copy: NULL pointer");
   if (bytes % 16 || (uintptr_t)src % 16 || (uintptr_t)dst % 16)

@@ -137,6 +137,14 @@ hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t s);
 // Copies n16 16-byte words with nontemporal loads and stores (bandwidth probe).
 hipError_t launch_copy_stream(void* dst, const void* src, size_t n16, hipStream_t s);
 
+// Waveguide mode solver (kernels_mode.hip): ww frequencies of a (xx, yy)
+// cross-section, 1 <= mm <= kModeMaxModes modes, one workgroup per frequency.
+constexpr int kModeMaxModes = 10;
+size_t mode_workspace_floats(int ww, int xx, int yy, int mm);
+hipError_t launch_mode(int ww, int xx, int yy, int mm, const float* eps, const float* omega,
+                       const float* shift, float* x, float* w, float* err, int* iters,
+                       int max_iters, float tol, float* ws, hipStream_t s);
+
 hipError_t run_reference(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                          const Src& src, const OutWin& win, char* ws, const Layout& L,
                          const StepRange& r, hipStream_t s);

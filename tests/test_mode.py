@@ -77,10 +77,87 @@ def test_full_fields(prop_axis):
       np.sum(e[:, 0] * h[:, 1] - e[:, 1] * h[:, 0], axis=(1, 2)), 1)
 
 
+def test_kernel_entry_points_reject_bad_arguments():
+  """The C ABI validates before launching (no GPU needed)."""
+  import ctypes
+  from pjz_amd import _lib
+  lib = _lib.load()
+  nb = ctypes.c_size_t()
+  assert lib.pjz_fdtd_mode_workspace_bytes(1, 30, 20, 4, ctypes.byref(nb)) == _lib.OK
+  assert nb.value == 2 * 2 * 30 * 20 * 4 * 4
+  for bad in [(0, 30, 20, 4), (1, 0, 20, 4), (1, 30, 20, 0), (1, 30, 20, M.HIP_MAX_MODES + 1)]:
+    assert lib.pjz_fdtd_mode_workspace_bytes(*bad, ctypes.byref(nb)) == _lib.EINVAL
+  args = [1, 30, 20, 4] + [None] * 7 + [10, 1e-4, None, 0, None]
+  assert lib.pjz_fdtd_mode_solve(*args) == _lib.EINVAL
+  assert "NULL" in lib.pjz_fdtd_last_error().decode()
+
+
 @pytest.mark.gpu
-def test_mode_runs_on_the_gpu(gpu):
-  eps = torch.as_tensor(strip("x"), device=gpu)
-  beta, field, err, iters = M.mode(eps, np.array([2 * np.pi / 37]), num_modes=4,
-                                   max_iters=20000)
-  assert field.device.type == "cuda"
+@pytest.mark.parametrize("prop_axis", ["x", "y", "z"])
+def test_gpu_kernel_golden_betas_and_init(gpu, prop_axis):
+  eps = torch.as_tensor(strip(prop_axis), device=gpu)
+  omega = np.array([2 * np.pi / 37])
+  beta, field, err, iters = M.mode(eps, omega, num_modes=4, max_iters=20000)
+  assert field.device.type == "cuda" and field.dtype == torch.float32
+  print(f"{prop_axis}: iters {iters} beta {beta[0].cpu().numpy()}", flush=True)
   assert beta[0].cpu().numpy() == pytest.approx(GOLDEN_BETAS, rel=1e-3)
+  b1, f1, _, _ = M.mode(eps, omega, num_modes=1)
+  b2, _, _, it2 = M.mode(eps, omega, init=f1, num_modes=1)
+  assert it2 == 1
+  np.testing.assert_array_almost_equal(b1.cpu().numpy(), b2.cpu().numpy(), decimal=3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prop_axis", ["x", "y", "z"])
+def test_gpu_kernel_full_fields(gpu, prop_axis):
+  ww, mm = 4, 2
+  omega = np.linspace(2 * np.pi / 37, 2 * np.pi / 36, ww)
+  eps = strip(prop_axis)
+  beta, field, _, _ = M.mode(torch.as_tensor(eps, device=gpu), omega, num_modes=mm,
+                             max_iters=20000)
+  field = field.cpu().numpy()
+  if prop_axis == "x":
+    f = np.flip(field[:, :, 0], axis=1)
+    eps = eps[(1, 2, 0), ...]
+  elif prop_axis == "y":
+    f = np.flip(np.swapaxes(field[:, :, :, 0], 2, 3), axis=2)
+    eps = np.flip(np.swapaxes(eps[(2, 0, 1), ...], 1, 3), axis=1)
+  else:
+    f = np.array([-1, 1])[None, :, None, None, None] * np.flip(field[:, :, :, :, 0], axis=1)
+  h, e, h2 = M.full_fields(beta.cpu().double(), omega, np.squeeze(eps), f.astype(np.float64))
+  h, e, h2 = (t.numpy() for t in (h, e, h2))
+  hflat, hflat2 = (x.reshape((ww, -1, mm)) for x in (h, h2))
+  np.testing.assert_array_less(
+      np.linalg.norm(hflat - hflat2, axis=1) / np.linalg.norm(hflat, axis=1), 1e-2)
+  np.testing.assert_array_almost_equal(
+      np.sum(e[:, 0] * h[:, 1] - e[:, 1] * h[:, 0], axis=(1, 2)), 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("xx,yy,mm,ww", [(30, 20, 1, 1), (17, 23, 3, 2), (40, 9, 9, 3),
+                                         (2, 31, 2, 1), (64, 48, 10, 2)])
+def test_gpu_kernel_matches_torch_iteration(gpu, xx, yy, mm, ww):
+  """A fixed number of iterations of the kernel and of the torch loop from
+  the same start vectors and shift give the same Rayleigh quotients (the QR
+  differs: CholeskyQR2 against Householder, so vectors agree up to sign)."""
+  rng = np.random.default_rng(xx * yy + mm)
+  eps = torch.tensor(1 + 11 * rng.random((3, xx, yy)), dtype=torch.float32, device=gpu)
+  omega = torch.tensor(2 * np.pi / (30 + 5 * rng.random(ww)), dtype=torch.float32, device=gpu)
+  x0 = M._random((ww, 2, xx, yy, mm), gpu, seed=7)
+  shift, _, _, _ = M._subspace_iteration(M._operator(eps, omega, torch.zeros_like(omega)),
+                                         M._random((ww, 2, xx, yy, 1), gpu), 10, 1e-4)
+  shift = shift[:, 0]
+  n = 40
+  w0, x0t, e0, i0 = M._subspace_iteration(M._operator(eps, omega, shift), x0, n, 0.0)
+  w1, x1, e1, i1 = M._subspace_iteration_hip(eps, omega, shift, x0, n, 0.0)
+  print(f"case {(xx, yy, mm, ww)}: w torch {w0[0].cpu().numpy()} hip {w1[0].cpu().numpy()}",
+        flush=True)
+  assert i0 == i1 == n
+  scale = float(torch.max(torch.abs(w0)))
+  assert float(torch.max(torch.abs(w1 - w0))) <= 2e-5 * scale
+  assert float(torch.max(torch.abs(e1 - e0))) <= 2e-4 * scale + 1e-3 * float(torch.max(e0))
+  # Orthonormal columns.
+  q = x1.reshape(ww, -1, mm)
+  gram = torch.einsum("wrk,wrl->wkl", q.double(), q.double())
+  eye = torch.eye(mm, dtype=torch.float64, device=gpu).expand(ww, mm, mm)
+  assert float(torch.max(torch.abs(gram - eye))) <= 1e-5
