@@ -500,8 +500,32 @@ static int device_cus2() {
   return cus;
 }
 
+// Workgroups of a kernel that fit on one CU at once (occupancy query): at
+// one cell per lane the 7-wave reverse kernel needs ~120 registers, so two
+// workgroups share a CU; at two cells per lane (~200) one does.
+template <typename K>
+static int adj2_resident(K kernel) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(kernel),
+                                                   64 * (kAdj2WY + 3), 0) != hipSuccess || b < 1)
+    b = 1;
+  return b;
+}
+
+template <int CPL>
+static int adj2_resident_single() {
+  static const int n = adj2_resident(&k_adj_stream2<CPL, kAdj2WY>);
+  return n;
+}
+
+template <int CPL>
+static int adj2_resident_batch() {
+  static const int n = adj2_resident(&k_adj_stream2_batch<CPL, kAdj2WY>);
+  return n;
+}
+
 // Arguments of a pair of reversed steps reading adjoint set acur; LX = 0
-// picks x chunks giving about one workgroup per CU.
+// picks x chunks giving about one tile per resident workgroup slot.
 static Adj2Args adj2_args(const Geom& g, const pjz_fdtd_problem& p, const float* mask, char* ws,
                           const Layout& L, const float* D1, const float* D2, int acur, int LX) {
   Adj2Args a;
@@ -531,7 +555,9 @@ static Adj2Args adj2_args(const Geom& g, const pjz_fdtd_problem& p, const float*
   a.dt = p.dt;
   a.nty = (g.Y + kAdj2WY - 1) / kAdj2WY;
   if (LX <= 0) {
-    const int ntx = std::max(1, (device_cus2() + a.nty - 1) / a.nty);
+    const int slots =
+        device_cus2() * (g.Z <= 64 ? adj2_resident_single<1>() : adj2_resident_single<2>());
+    const int ntx = std::max(1, (slots + a.nty - 1) / a.nty);
     LX = std::max(8, (g.X + ntx - 1) / ntx);
   }
   a.LX = LX;
@@ -625,7 +651,8 @@ hipError_t launch_adjoint2_batch_step(const Adj2BatchPlan& plan, int launch, int
   for (int c = 0; c < 2; ++c) {
     const int nc = plan.count[c];
     if (nc == 0) continue;
-    const int grid = std::min(plan.total[c], plan.cus);
+    const int grid = std::min(
+        plan.total[c], plan.cus * (c == 0 ? adj2_resident_batch<1>() : adj2_resident_batch<2>()));
     const Adj2Args* a = dargs + 2 * plan.base[c];
     const int* ts = dts + plan.base[c] + c;
     const int* ls = dls + plan.base[c];

@@ -710,9 +710,11 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
                        const StepRange& r, hipStream_t s) {
   if (!stream2_supported(g)) return hipErrorInvalidValue;
   const int cpl = g.Z <= 64 ? 1 : (g.Z <= 128 ? 2 : 4);
-  // Row tiles with a compiled kernel: 8 (4 at CPL = 4); CPL = 2 also 4, 12, 13.
+  // Row tiles with a compiled kernel: 8 (4 at CPL = 4); CPL = 2 also 4, 12,
+  // 13; CPL = 1 also 13 (16 waves, which its register use lets share a CU).
   int WY = cpl == 4 ? 4 : 8;
   if (cpl == 2 && (p.tile_y == 4 || p.tile_y == 12 || p.tile_y == 13)) WY = p.tile_y;
+  if (cpl == 1 && p.tile_y == 13) WY = 13;
   int LX = p.tile_x;
   if (LX <= 0) {
     // x chunks sized so the grid has about one workgroup per CU.
@@ -745,7 +747,8 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
     a.n = n;
     a.flags = (slot(n) >= 0 ? SNAP1 : 0) | (slot(n + 1) >= 0 ? SNAP2 : 0) |
               (r.capture ? CAPTURE : 0);
-    if (cpl == 1) launch2<1, 8, true>(a, s);
+    if (cpl == 1 && WY == 13) launch2<1, 13, true>(a, s);
+    else if (cpl == 1) launch2<1, 8, true>(a, s);
     else if (cpl == 4) launch2<4, 4, true>(a, s);
     else if (WY == 4) launch2<2, 4, true>(a, s);
     else if (WY == 12) launch2<2, 12, true>(a, s);
@@ -766,8 +769,29 @@ bool batch2_supported(const Geom& g) {
   return fits_rsrc(g) && (g.Z <= 64 || (g.Z <= 128 && g.Z % 2 == 0));
 }
 
-static constexpr int kBatchWY = 8;
+// Rows per batch tile by cells-per-lane class: zz <= 64 (CPL = 1) takes 13
+// rows (16 waves at ~120 registers, one workgroup filling a CU), zz <= 128
+// takes 8 (11 waves at 168 registers).
+static constexpr int kBatchWY1 = 13;
+static constexpr int kBatchWY2 = 8;
 static constexpr int kBatchLX = 128;
+
+// Workgroups of a persistent kernel that fit on one CU at once (occupancy
+// query, once per kernel): the persistent grid is that many per CU.
+template <typename K>
+static int resident_per_cu(K kernel, int threads) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(kernel),
+                                                   threads, 0) != hipSuccess || b < 1)
+    b = 1;
+  return b;
+}
+
+template <int CPL, int WY>
+static int batch2_resident() {
+  static const int n = resident_per_cu(&k_stream2_batch<CPL, WY, true>, 64 * (WY + 3));
+  return n;
+}
 static constexpr int kMaxChunkPairs = 64;  // step pairs per captured graph chunk
 
 size_t batch2_meta_bytes(const BatchItem* items, int n) {
@@ -814,10 +838,11 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
     for (size_t k = 0; k < cls[c].size(); ++k) {
       const BatchItem& it = items[cls[c][k]];
       const int LX = std::min(kBatchLX, it.g.X);
+      const int WY = c == 0 ? kBatchWY1 : kBatchWY2;
       hrare[off + k] = rare_of(*it.p, it.win, it.capture, 0);
       for (int cur = 0; cur < 2; ++cur) {
         hargs[2 * (off + k) + cur] =
-            make_args2(it.g, *it.p, it.mask, it.src, it.win, it.ws, *it.L, kBatchWY, LX, cur);
+            make_args2(it.g, *it.p, it.mask, it.src, it.win, it.ws, *it.L, WY, LX, cur);
         hargs[2 * (off + k) + cur].rare = drare + off + k;
       }
       hts[off + c + k] = t;
@@ -842,17 +867,19 @@ static void launch_pair(const Batch2Plan& plan, hipStream_t s, int* ctr, const i
   for (int c = 0; c < 2; ++c) {
     const int nc = plan.count[c];
     if (nc == 0) continue;
-    const int grid = std::min(plan.total[c], plan.cus);
     const Stream2Args* a = dargs + 2 * plan.base[c];
     const int* ts = dts + plan.base[c] + c;
-    if (c == 0)
-      hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY, true>), dim3(grid),
-                         dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,
+    if (c == 0) {
+      const int grid = std::min(plan.total[c], plan.cus * batch2_resident<1, kBatchWY1>());
+      hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY1, true>), dim3(grid),
+                         dim3(64 * (kBatchWY1 + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,
                          step_base, step_off);
-    else
-      hipLaunchKernelGGL((k_stream2_batch<2, kBatchWY, true>), dim3(grid),
-                         dim3(64 * (kBatchWY + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,
+    } else {
+      const int grid = std::min(plan.total[c], plan.cus * batch2_resident<2, kBatchWY2>());
+      hipLaunchKernelGGL((k_stream2_batch<2, kBatchWY2, true>), dim3(grid),
+                         dim3(64 * (kBatchWY2 + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,
                          step_base, step_off);
+    }
   }
 }
 
