@@ -137,6 +137,10 @@ hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t s);
 // Copies n16 16-byte words with nontemporal loads and stores (bandwidth probe).
 hipError_t launch_copy_stream(void* dst, const void* src, size_t n16, hipStream_t s);
 
+// x chunk length for a batch class: minimises rounds x planes per tile over
+// candidate lengths <= max_lx (xs, nty: each simulation's X and row tiles).
+int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx);
+
 // Waveguide mode solver (kernels_mode.hip): ww frequencies of a (xx, yy)
 // cross-section, 1 <= mm <= kModeMaxModes modes, one workgroup per frequency.
 constexpr int kModeMaxModes = 10;

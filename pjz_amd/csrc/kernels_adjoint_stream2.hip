@@ -612,11 +612,19 @@ hipError_t adjoint2_batch_prepare(const BatchItem* items, int n, int S, const in
   for (int c = 0; c < 2; ++c) {
     plan.base[c] = off;
     plan.count[c] = (int)cls[c].size();
+    std::vector<int> xs, ntys;
+    for (int i : cls[c]) {
+      xs.push_back(items[i].g.X);
+      ntys.push_back((items[i].g.Y + kAdj2WY - 1) / kAdj2WY);
+    }
+    const int slots =
+        plan.cus * (c == 0 ? adj2_resident_batch<1>() : adj2_resident_batch<2>());
+    const int cLX = batch_chunk_x(xs.data(), ntys.data(), (int)xs.size(), slots, kAdj2BatchLX);
     int t = 0;
     for (size_t k = 0; k < cls[c].size(); ++k) {
       const int i = cls[c][k];
       const BatchItem& it = items[i];
-      const int LX = std::min(kAdj2BatchLX, it.g.X);
+      const int LX = std::min(cLX, it.g.X);
       for (int v = 0; v < 2; ++v)
         hargs[2 * (off + k) + v] =
             adj2_args(it.g, *it.p, it.mask, it.ws, *it.L, it.capture, it.capture, v, LX);

@@ -22,6 +22,8 @@
 // at a fixed offset from E, so few pointers and no 64-bit index products stay
 // live in SGPRs.
 #include <algorithm>
+#include <functional>
+#include <queue>
 #include <vector>
 
 #include "stream_common.h"
@@ -792,6 +794,44 @@ static int batch2_resident() {
   static const int n = resident_per_cu(&k_stream2_batch<CPL, WY, true>, 64 * (WY + 3));
   return n;
 }
+
+// x chunk length of a batch's tiles.  The persistent workgroups take tiles
+// from the queue in order (simulation by simulation, tile t of a simulation
+// at row tile t % nty, x chunk t / nty), each taking about min(LX, rest of
+// X) + 2 planes of time (two ghost planes).  Simulate that greedy schedule
+// on `slots` workgroups for each candidate LX and keep the longest chunk
+// within 3% of the shortest makespan (longer chunks re-read fewer ghost
+// planes, which the model does not charge).
+int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx) {
+  static const int cands[] = {128, 96, 64, 48, 32};
+  std::vector<long long> cost;
+  std::vector<int> lxs;
+  for (int lx : cands) {
+    if (lx > max_lx) continue;
+    std::priority_queue<long long, std::vector<long long>, std::greater<long long>> free_at;
+    for (int w = 0; w < slots; ++w) free_at.push(0);
+    long long span = 0;
+    for (int i = 0; i < n; ++i) {
+      const int ntx = (xs[i] + lx - 1) / lx;
+      for (int tx = 0; tx < ntx; ++tx) {
+        const long long d = std::min(lx, xs[i] - tx * lx) + 2;
+        for (int ty = 0; ty < nty[i]; ++ty) {
+          const long long t = free_at.top() + d;
+          free_at.pop();
+          free_at.push(t);
+          span = std::max(span, t);
+        }
+      }
+    }
+    cost.push_back(span);
+    lxs.push_back(lx);
+  }
+  if (lxs.empty()) return max_lx;
+  const long long best = *std::min_element(cost.begin(), cost.end());
+  for (size_t c = 0; c < lxs.size(); ++c)  // candidates run longest first
+    if (cost[c] * 100 <= best * 103) return lxs[c];
+  return lxs[0];
+}
 static constexpr int kMaxChunkPairs = 64;  // step pairs per captured graph chunk
 
 size_t batch2_meta_bytes(const BatchItem* items, int n) {
@@ -834,11 +874,19 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
   for (int c = 0; c < 2; ++c) {
     plan.base[c] = off;
     plan.count[c] = (int)cls[c].size();
+    const int WY = c == 0 ? kBatchWY1 : kBatchWY2;
+    std::vector<int> xs, ntys;
+    for (int i : cls[c]) {
+      xs.push_back(items[i].g.X);
+      ntys.push_back((items[i].g.Y + WY - 1) / WY);
+    }
+    const int slots = plan.cus * (c == 0 ? batch2_resident<1, kBatchWY1>()
+                                         : batch2_resident<2, kBatchWY2>());
+    const int cLX = batch_chunk_x(xs.data(), ntys.data(), (int)xs.size(), slots, kBatchLX);
     int t = 0;
     for (size_t k = 0; k < cls[c].size(); ++k) {
       const BatchItem& it = items[cls[c][k]];
-      const int LX = std::min(kBatchLX, it.g.X);
-      const int WY = c == 0 ? kBatchWY1 : kBatchWY2;
+      const int LX = std::min(cLX, it.g.X);
       hrare[off + k] = rare_of(*it.p, it.win, it.capture, 0);
       for (int cur = 0; cur < 2; ++cur) {
         hargs[2 * (off + k) + cur] =
