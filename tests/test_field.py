@@ -64,3 +64,19 @@ def test_field_matches_oracle(axis, gpu):
   err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
   print(f"field {axis}: rel L2 {err:.2e}")
   assert err <= 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("z_batch", [False, True])
+def test_field_reduced_precision_default(z_batch, gpu):
+  """pjz's default use_reduced_precision=True gives the larger z budget
+  (128 - PML cells, _field.py:56-58); z as batch switches the CPML off."""
+  eps, src, pos = problem("x", seed=2)
+  p = P._replace(use_reduced_precision=True, use_z_as_batch=z_batch)
+  omega = np.array([2 * np.pi / 20])
+  got = Fd.field(torch.as_tensor(eps, device=gpu), torch.as_tensor(src, device=gpu), omega,
+                 pos, p).cpu().numpy()
+  ref = field_ref.field(eps, src, omega, pos, p)
+  err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+  print(f"field reduced z_batch={z_batch}: rel L2 {err:.2e}", flush=True)
+  assert err <= 1e-4
