@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import boundaries as B
-from .fdtdz_torch import fdtdz
+from .fdtdz_torch import fdtdz, fdtdz_batch
 
 
 class SimParams(NamedTuple):
@@ -189,3 +189,29 @@ def field(epsilon, source, omega, source_pos, sim_params):
   kw, output_steps = simulation_inputs(epsilon, source, omega, source_pos, sim_params)
   fields = fdtdz(**kw)
   return project(fields, omega, output_steps, sim_params.dt)
+
+
+def _batchable(zz):
+  """Whether the batched kernels take a padded domain height of zz."""
+  return zz <= 64 or (zz <= 128 and zz % 2 == 0)
+
+
+def fields(epsilon, sources, omega, source_positions, sim_params):
+  """``field()`` for several sources on one structure (pjz's per-port loop
+  in ``scatter()``, ``_field.py:354-356``), advanced together by one batched
+  simulation (:func:`pjz_amd.fdtdz_batch`) when the padded domain height
+  allows, otherwise one simulation per source.  Returns a list of
+  (ww, 3, xx, yy, zz) complex64 fields, equal to ``field()`` per source.
+  """
+  dev = epsilon.device if isinstance(epsilon, torch.Tensor) and epsilon.is_cuda else \
+      torch.device("cuda", torch.cuda.current_device()) if torch.cuda.is_available() else None
+  if dev is None:
+    raise RuntimeError("fields() needs a HIP GPU: torch.cuda.is_available() is False")
+  epsilon = _as_tensor(epsilon, dev, torch.float32)
+  runs = [simulation_inputs(epsilon, _as_tensor(src, dev), omega, pos, sim_params)
+          for src, pos in zip(sources, source_positions)]
+  if len(runs) > 1 and all(_batchable(kw["pml_sigma"].shape[0]) for kw, _ in runs):
+    outs = fdtdz_batch([kw for kw, _ in runs])
+  else:
+    outs = [fdtdz(**kw) for kw, _ in runs]
+  return [project(f, omega, steps, sim_params.dt) for f, (_, steps) in zip(outs, runs)]

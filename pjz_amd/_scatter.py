@@ -6,14 +6,15 @@ planes next to each port split into forward/backward amplitudes, S-values
 normalised by the injected amplitude, and a custom backward pass that
 returns pjz's reciprocity gradient  sum_ij Re(g_ij * f_i * f_j / a_i)
 (``_field.py:380-382,393-398``) instead of differentiating through the
-simulations.  Runs on the GPU through :func:`pjz_amd.field.field`.
+simulations.  The per-port simulations run as one batch on the GPU
+(:func:`pjz_amd._field.fields`).
 """
 
 from typing import Any, Sequence
 
 import torch
 
-from ._field import SimParams, field, prop_axis, transverse_slice
+from ._field import SimParams, fields as port_fields, prop_axis, transverse_slice
 
 
 def amplitudes(beta, vals, x):
@@ -51,8 +52,9 @@ def overlap(mode, beta, pos, is_fwd, output):
 
 
 def _impl(epsilon, omega, modes, betas, pos, is_fwd, sim_params):
-  fields = [field(epsilon, torch.mean(m, dim=0), omega, p, sim_params)
-            for m, p in zip(modes, pos)]
+  # One simulation per port (source = the port mode averaged over frequency),
+  # all ports advanced together by the batched kernel.
+  fields = port_fields(epsilon, [torch.mean(m, dim=0) for m in modes], omega, pos, sim_params)
   amps = [torch.ones_like(b, dtype=torch.complex128) if fwd is None
           else overlap(m, b, p, fwd, f)[:, 0]
           for f, m, b, p, fwd in zip(fields, modes, betas, pos, is_fwd)]

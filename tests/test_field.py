@@ -80,3 +80,21 @@ def test_field_reduced_precision_default(z_batch, gpu):
   err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
   print(f"field reduced z_batch={z_batch}: rel L2 {err:.2e}", flush=True)
   assert err <= 1e-4
+
+
+@pytest.mark.gpu
+def test_fields_batch_equals_field_per_source(gpu):
+  """fields() (scatter's per-port loop as one batch) gives field() per source."""
+  eps, _, _ = problem("x", seed=3)
+  rng = np.random.default_rng(4)
+  srcs = [torch.as_tensor(rng.standard_normal((2, 1, 12, 20)).astype(np.float32), device=gpu)
+          for _ in range(3)]
+  positions = [4, 7, 9]
+  omega = np.array([2 * np.pi / 21, 2 * np.pi / 19])
+  e = torch.as_tensor(eps, device=gpu)
+  got = Fd.fields(e, srcs, omega, positions, P)
+  for g, s, p in zip(got, srcs, positions):
+    ref = Fd.field(e, s, omega, p, P)
+    err = float(torch.linalg.vector_norm(g - ref) / torch.linalg.vector_norm(ref))
+    print(f"port at {p}: rel {err:.2e}", flush=True)
+    assert err <= 1e-6
