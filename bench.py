@@ -263,6 +263,50 @@ def bench_design(args):
   print(json.dumps(line), flush=True)
 
 
+def bench_mode(args):
+  """pjz's mode solver: microseconds per subspace iteration of the HIP kernel
+  (one launch runs the loop) against the same iteration as torch ops on the
+  same GPU, on the 30 x 20 strip of pjz's tests (4 modes) and a 256 x 128
+  section (2 modes), `tol = 0` so both run exactly the iteration count."""
+  import numpy as np
+  import torch
+  from pjz_amd import _mode as M
+  dev = torch.device("cuda", 0)
+  iters = args.time_steps or 2000
+  rows = []
+  for (uu, vv, mm) in ((30, 20, 4), (256, 128, 2)):
+    eps = np.ones((3, uu, vv))
+    eps[:, uu * 3 // 10: uu * 7 // 10, vv * 2 // 5: vv * 3 // 5] = 12.25
+    e = torch.as_tensor(eps[:, None], device=dev)
+    omega = np.array([2 * np.pi / 37])
+    res = {}
+    for impl in ("hip", "torch"):
+      limit = M.HIP_MAX_MODES
+      if impl == "torch":
+        M.HIP_MAX_MODES = 0
+      try:
+        M.mode(e, omega, num_modes=mm, max_iters=20)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        beta, _, _, it = M.mode(e, omega, num_modes=mm, max_iters=iters, tol=0.0)
+        torch.cuda.synchronize()
+        res[impl] = ((time.perf_counter() - t0) / it * 1e6, beta[0].cpu().numpy())
+      finally:
+        M.HIP_MAX_MODES = limit
+    rows.append({"section": f"{uu}x{vv}", "modes": mm, "iterations": iters,
+                 "hip_us_per_iter": round(res["hip"][0], 2),
+                 "torch_us_per_iter": round(res["torch"][0], 2),
+                 "beta_max_rel_diff": float(np.max(np.abs(res["hip"][1] - res["torch"][1]) /
+                                                   np.abs(res["torch"][1])))})
+  line = {"metric": "mode solver subspace iterations per second", "unit": "iterations/s",
+          "value": round(1e6 / rows[0]["hip_us_per_iter"], 1), "n_gpus": 1, "steps": 1,
+          "warmup": 1, "ms_per_step": round(rows[0]["hip_us_per_iter"] * iters / 1e3, 3),
+          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+          "data": "synthetic", "config": {"workload": "mode: pjz mode() subspace iteration",
+                                          "cases": rows}}
+  print(json.dumps(line), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -285,6 +329,8 @@ def main():
     return bench_batch(args)
   if args.workload == "design_batch":
     return bench_design(args)
+  if args.workload == "mode":
+    return bench_mode(args)
 
   import numpy as np
   import torch
