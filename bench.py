@@ -62,15 +62,40 @@ def cpu_baseline(kw, target_s=10.0):
 
   # Set-up (coefficients, allocation) is timed at 0 steps and subtracted.
   t_setup = run(0)
-  t_probe = run(3) - t_setup
-  n = int(max(3, min(2000, 3 * target_s / max(t_probe, 1e-3))))
+  t_probe = run(10) - t_setup
+  n = int(max(10, min(4000, 10 * target_s / max(t_probe, 1e-3))))
   wall = run(n) - t_setup
+  # Host roofline: the stepper's FP64 algorithmic bytes per cell-update (one
+  # step per pass: twice the fp32 figure) against a multi-threaded 1 GiB host
+  # copy on the same cores.
+  lo, hi = kw["pml_widths"]
+  bpc64 = 2.0 * algorithmic_bytes_per_cell(kw["pml_sigma"].shape[0], lo, hi)
+  copy_gbs = host_copy_gbs(threads)
+  achieved = cells * n * bpc64 / wall / 1e9
   return {"value": round(cells * n / wall / 1e9, 5), "unit": "Gcell-updates/s",
           "cores": threads, "kind": "port",
           "sample": f"FP64 C oracle (oracle/fdtd_ref.c, OpenMP) on the full "
                     f"{kw['epsilon'].shape[1]}x{kw['epsilon'].shape[2]}x"
                     f"{kw['epsilon'].shape[3]} grid, {n} steps in {wall:.1f} s "
-                    f"(set-up of {t_setup:.1f} s excluded)"}
+                    f"(set-up of {t_setup:.1f} s excluded)",
+          "algorithmic_gbs": round(achieved, 2), "host_copy_gbs": round(copy_gbs, 2),
+          "roofline_frac": round(achieved / copy_gbs, 4)}
+
+
+def host_copy_gbs(threads, gib=1, reps=3):
+  """Host DRAM bandwidth (GB/s, read + write): best of `reps` torch copies of
+  `gib` GiB on `threads` threads."""
+  import torch
+  torch.set_num_threads(threads)
+  a = torch.ones((gib << 30) // 4, dtype=torch.float32)
+  b = torch.empty_like(a)
+  b.copy_(a)
+  best = 1e30
+  for _ in range(reps):
+    t0 = time.perf_counter()
+    b.copy_(a)
+    best = min(best, time.perf_counter() - t0)
+  return 2.0 * (gib << 30) / best / 1e9
 
 
 def copy_ceiling_gbs(dev, gib=1, reps=10):

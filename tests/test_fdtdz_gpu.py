@@ -164,6 +164,17 @@ def test_stream_launch_variants_match_oracle(tile_x, kernel, tile_y):
   assert rel_l2(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("tile_y", [8, 13])
+@pytest.mark.parametrize("Z", [40, 64])
+def test_one_cell_per_lane_row_tiles_match_oracle(tile_y, Z):
+  """zz <= 64 (one cell per lane): the 8- and 13-row two-step tiles."""
+  kw = random_problem(1, X=33, Y=29, Z=Z, tt=47, seed=12, widths=(5, 6))
+  ref = C.fdtdz(**kw)
+  kw["launch_params"] = dict(kernel="stream2", tile_y=tile_y)
+  got = F.fdtdz(**kw).cpu().numpy()
+  assert rel_l2(got, ref) <= TOL
+
+
 @pytest.mark.parametrize("Z", [32, 64, 96])
 def test_autotune_matches_oracle_and_reuses_choice(Z):
   """The autotuned call (first call times the candidates on the caller's
