@@ -4,12 +4,16 @@
 //
 // One workgroup per frequency owns that frequency's whole iteration, so the
 // host never synchronises inside the loop (a torch version of the same loop
-// is bound by ~30 launches and one host read per iteration).  Per iteration:
-//   1. x = orthonormal basis of y (CholeskyQR2: Gram matrix in double,
-//      Cholesky and the triangular inverse on one lane, x = y R^-1, twice);
-//   2. y = A x, the 5-point operator in two passes (the scalar fields
-//      b = (d-y u0 ... ) / eps_z and c = div u, then the vector update);
-//   3. w_k = <x_k, y_k>, err_k = |y_k - w_k x_k| (double sums).
+// is bound by ~30 launches and one host read per iteration).  Per iteration,
+// four passes over the vectors:
+//   1. x = y R1^-1, summing the Gram matrix of x as the rows are written;
+//   2. x = x R2^-1 (1 and 2 are CholeskyQR2: Gram matrices in double,
+//      Cholesky and the triangular inverse on one lane);
+//   3. y = A x, first stencil pass: the scalar fields b = (-(d-y u0) +
+//      d-x u1) / eps_z and c = d+x u0 + d+y u1;
+//   4. second stencil pass, the vector update, summing w_k = <x_k, y_k>,
+//      |y_k|^2, |x_k|^2 (so err_k = |y_k - w_k x_k|) and the Gram matrix of
+//      y, which gives the next iteration's R1.
 // The loop stops after max_iters or once every err_k <= tol, the test pjz
 // makes before each iteration.  Vectors are (2, XX, YY, MM) with the MM modes
 // innermost, as pjz lays them out; the periodic differences wrap in x and y
@@ -22,7 +26,6 @@ namespace pjz {
 
 constexpr int kModeThreads = 512;
 constexpr int kModeWaves = kModeThreads / 64;
-constexpr int kChunk = 16;  // doubles per block reduction
 
 struct ModeArgs {
   int XX, YY, max_iters;
@@ -42,31 +45,32 @@ template <int MM>
 struct ModeLds {
   static constexpr int NPAIR = MM * (MM + 1) / 2;  // Gram entries (upper triangle)
   static constexpr int NQ = 3 * MM;                // <x,y>, <y,y>, <x,x> per mode
-  double red[kModeWaves][kChunk];
-  double sum[NPAIR > NQ ? NPAIR : NQ];
+  static constexpr int NALL = NQ + NPAIR;
+  double red[kModeWaves][NALL];
+  double sum[NALL];    // [NQ Rayleigh sums | NPAIR Gram entries]
   double R[MM * MM];   // Cholesky factor, upper (G = R^T R)
   double Ri[MM * MM];  // its inverse, upper
   int go;
 };
 
-// Sums acc[0..P) over the workgroup into out[0..P) (LDS); fixed order.
-template <int MM>
-__device__ inline void block_sum(const double (&acc)[kChunk], int P, ModeLds<MM>& L, double* out) {
+// Sums acc[0..P) over the workgroup into out[0..P) (LDS): wavefront
+// reductions, one LDS row per wave, one barrier, the cross-wave sums, one
+// barrier.  Fixed order, so results do not depend on timing.
+template <int MM, int P>
+__device__ __forceinline__ void block_sum(const double (&acc)[P], ModeLds<MM>& L, double* out) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int p = 0; p < kChunk; ++p) {
-    if (p < P) {
-      double v = acc[p];
+  for (int p = 0; p < P; ++p) {
+    double v = acc[p];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      if (lane == 0) L.red[wv][p] = v;
-    }
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) L.red[wv][p] = v;
   }
   __syncthreads();
-  if ((int)threadIdx.x < P) {
-    double s = 0.0;
-    for (int q = 0; q < kModeWaves; ++q) s += L.red[q][threadIdx.x];
-    out[threadIdx.x] = s;
+  for (int q = threadIdx.x; q < P; q += kModeThreads) {
+    double t = 0.0;
+    for (int w = 0; w < kModeWaves; ++w) t += L.red[w][q];
+    out[q] = t;
   }
   __syncthreads();
 }
@@ -75,86 +79,68 @@ __device__ inline void block_sum(const double (&acc)[kChunk], int P, ModeLds<MM>
 template <int MM>
 __device__ constexpr int tri(int a, int b) { return a * MM - a * (a - 1) / 2 + (b - a); }
 
-// G = v^T v over the n rows of v (MM columns) into L.sum (upper triangle),
-// then, on one lane, R (G = R^T R) and Ri = R^-1 in LDS.  The loops over
-// modes unroll completely, so every register index is a constant.
-template <int MM>
-__device__ void gram_chol(const float* __restrict__ v, int n, ModeLds<MM>& L) {
-  constexpr int NPAIR = ModeLds<MM>::NPAIR;
+// acc[tri(a, b)] += v[a] v[b] for a <= b (the loops unroll completely, so
+// every register index is a constant).
+template <int MM, int P>
+__device__ __forceinline__ void add_gram(double (&acc)[P], int off, const float (&v)[MM]) {
 #pragma unroll
-  for (int pc = 0; pc < NPAIR; pc += kChunk) {
-    double acc[kChunk];
+  for (int a = 0; a < MM; ++a) {
 #pragma unroll
-    for (int p = 0; p < kChunk; ++p) acc[p] = 0.0;
-    for (int r = threadIdx.x; r < n; r += kModeThreads) {
-      float row[MM];
-#pragma unroll
-      for (int k = 0; k < MM; ++k) row[k] = v[r * MM + k];
-#pragma unroll
-      for (int a = 0; a < MM; ++a) {
-#pragma unroll
-        for (int b = a; b < MM; ++b) {
-          const int q = tri<MM>(a, b) - pc;
-          if (q >= 0 && q < kChunk) acc[q] += (double)row[a] * (double)row[b];
-        }
-      }
-    }
-    block_sum<MM>(acc, min(kChunk, NPAIR - pc), L, L.sum + pc);
+    for (int b = a; b < MM; ++b) acc[off + tri<MM>(a, b)] += (double)v[a] * (double)v[b];
   }
-  if (threadIdx.x == 0) {
-    double* R = L.R;
-    double* Ri = L.Ri;
-    for (int i = 0; i < MM * MM; ++i) R[i] = Ri[i] = 0.0;
-    for (int j = 0; j < MM; ++j) {
-      double d = L.sum[tri<MM>(j, j)];
-      for (int k = 0; k < j; ++k) d -= R[k * MM + j] * R[k * MM + j];
-      // A start vector inside the span of the others (degenerate input
-      // only) keeps a tiny pivot instead of a NaN.
-      R[j * MM + j] = sqrt(d > 1e-300 ? d : 1e-300);
-      for (int i = j + 1; i < MM; ++i) {
-        double t = L.sum[tri<MM>(j, i)];
-        for (int k = 0; k < j; ++k) t -= R[k * MM + j] * R[k * MM + i];
-        R[j * MM + i] = t / R[j * MM + j];
-      }
-    }
-    for (int j = 0; j < MM; ++j) {
-      Ri[j * MM + j] = 1.0 / R[j * MM + j];
-      for (int i = j - 1; i >= 0; --i) {
-        double t = 0.0;
-        for (int k = i + 1; k <= j; ++k) t += R[i * MM + k] * Ri[k * MM + j];
-        Ri[i * MM + j] = -t / R[i * MM + i];
-      }
-    }
-  }
-  __syncthreads();
 }
 
-// dst = src Ri, row by row (dst may alias src).
+// On one lane: R with G = R^T R for the Gram matrix in g (upper triangle),
+// and Ri = R^-1, into LDS.  The caller synchronises.
 template <int MM>
-__device__ void apply_ri(const float* src, float* dst, int n, const ModeLds<MM>& L) {
-  for (int r = threadIdx.x; r < n; r += kModeThreads) {
-    float row[MM];
-#pragma unroll
-    for (int k = 0; k < MM; ++k) row[k] = src[r * MM + k];
-#pragma unroll
-    for (int k = 0; k < MM; ++k) {
+__device__ __forceinline__ void chol_inv(const double* g, ModeLds<MM>& L) {
+  double* R = L.R;
+  double* Ri = L.Ri;
+  for (int i = 0; i < MM * MM; ++i) R[i] = Ri[i] = 0.0;
+  for (int j = 0; j < MM; ++j) {
+    double d = g[tri<MM>(j, j)];
+    for (int k = 0; k < j; ++k) d -= R[k * MM + j] * R[k * MM + j];
+    // A start vector inside the span of the others (degenerate input only)
+    // keeps a tiny pivot instead of a NaN.
+    R[j * MM + j] = sqrt(d > 1e-300 ? d : 1e-300);
+    for (int i = j + 1; i < MM; ++i) {
+      double t = g[tri<MM>(j, i)];
+      for (int k = 0; k < j; ++k) t -= R[k * MM + j] * R[k * MM + i];
+      R[j * MM + i] = t / R[j * MM + j];
+    }
+  }
+  for (int j = 0; j < MM; ++j) {
+    Ri[j * MM + j] = 1.0 / R[j * MM + j];
+    for (int i = j - 1; i >= 0; --i) {
       double t = 0.0;
-#pragma unroll
-      for (int j = 0; j <= k; ++j) t += (double)row[j] * L.Ri[j * MM + k];
-      dst[r * MM + k] = (float)t;
+      for (int k = i + 1; k <= j; ++k) t += R[i * MM + k] * Ri[k * MM + j];
+      Ri[i * MM + j] = -t / R[i * MM + i];
     }
   }
-  __syncthreads();
 }
 
-// y = A x for one frequency (om2 = omega^2, s = shift): _operator of
-// src/pjz/_mode.py:22-50, with the same grouping of the float operations.
+// One row times Ri (upper triangular).
 template <int MM>
-__device__ void apply_op(const float* __restrict__ x, float* __restrict__ y, float* __restrict__ bc,
-                         const float* __restrict__ eps, int XX, int YY, float om2, float s) {
+__device__ __forceinline__ void row_ri(float (&v)[MM], const ModeLds<MM>& L) {
+  float o[MM];
+#pragma unroll
+  for (int k = 0; k < MM; ++k) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j <= k; ++j) t += (double)v[j] * L.Ri[j * MM + k];
+    o[k] = (float)t;
+  }
+#pragma unroll
+  for (int k = 0; k < MM; ++k) v[k] = o[k];
+}
+
+// First stencil pass of y = A x (_operator, src/pjz/_mode.py:22-50): the
+// scalar fields b = (-(d-y u0) + d-x u1) / eps_z (backward differences) and
+// c = d+x u0 + d+y u1 (forward differences), periodic in x and y.
+template <int MM>
+__device__ __forceinline__ void op_pass1(const float* __restrict__ x, float* __restrict__ bc,
+                         const float* __restrict__ eps, int XX, int YY) {
   const int P = XX * YY, cells = P * MM;
-  const float* ex = eps;
-  const float* ey = eps + P;
   const float* ez = eps + 2 * P;
   for (int e = threadIdx.x; e < cells; e += kModeThreads) {
     const int p = e / MM, k = e - p * MM;
@@ -164,33 +150,99 @@ __device__ void apply_op(const float* __restrict__ x, float* __restrict__ y, flo
     const float* x0 = x;
     const float* x1 = x + cells;
     const float u0 = x0[e], u1 = x1[e];
-    const float b = (-(u0 - x0[(i * YY + jm) * MM + k]) + (u1 - x1[(im * YY + j) * MM + k])) / ez[p];
-    const float c = (x0[(ip * YY + j) * MM + k] - u0) + (x1[(i * YY + jp) * MM + k] - u1);
-    bc[e] = b;
-    bc[cells + e] = c;
+    bc[e] = (-(u0 - x0[(i * YY + jm) * MM + k]) + (u1 - x1[(im * YY + j) * MM + k])) / ez[p];
+    bc[cells + e] = (x0[(ip * YY + j) * MM + k] - u0) + (x1[(i * YY + jp) * MM + k] - u1);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < cells; e += kModeThreads) {
-    const int p = e / MM, k = e - p * MM;
+}
+
+// Whether the second stencil pass also sums the Gram matrix of y: its
+// 3 MM + MM (MM + 1) / 2 double accumulators fit in registers up to 6 modes;
+// above, a separate pass sums the Gram matrix.
+template <int MM>
+constexpr bool kFuseGram = MM <= 6;
+
+// Second stencil pass, one cell (both components, all modes) per thread, with
+// the same grouping of the float operations as _operator: y = A x, and into
+// acc the Rayleigh sums (x.y, y.y, x.x per mode) and, when P has room for it,
+// the Gram matrix of y.
+template <int MM, int P>
+__device__ __forceinline__ void op_pass2(const float* __restrict__ x, float* __restrict__ y,
+                         const float* __restrict__ bc, const float* __restrict__ eps, int XX,
+                         int YY, float om2, float s, double (&acc)[P]) {
+  const int PC = XX * YY, cells = PC * MM;
+  const float* ex = eps;
+  const float* ey = eps + PC;
+  const float* bb = bc;
+  const float* cc = bc + cells;
+  for (int p = threadIdx.x; p < PC; p += kModeThreads) {
     const int i = p / YY, j = p - i * YY;
     const int im = i == 0 ? XX - 1 : i - 1, ip = i == XX - 1 ? 0 : i + 1;
     const int jm = j == 0 ? YY - 1 : j - 1, jp = j == YY - 1 ? 0 : j + 1;
-    const float* bb = bc;
-    const float* cc = bc + cells;
-    const float b = bb[e], c = cc[e];
-    const float u0 = x[e], u1 = x[cells + e];
-    const float a0 = (om2 * ey[p] - s) * u0, a1 = (om2 * ex[p] - s) * u1;
-    const float b0 = -(bb[(i * YY + jp) * MM + k] - b) * ey[p];
-    const float b1 = (bb[(ip * YY + j) * MM + k] - b) * ex[p];
-    y[e] = (a0 + b0) + (c - cc[(im * YY + j) * MM + k]);
-    y[cells + e] = (a1 + b1) + (c - cc[(i * YY + jm) * MM + k]);
+    const float exv = ex[p], eyv = ey[p];
+    float x0[MM], x1[MM], y0[MM], y1[MM];
+#pragma unroll
+    for (int k = 0; k < MM; ++k) {
+      const int e = p * MM + k;
+      const float b = bb[e], c = cc[e];
+      x0[k] = x[e];
+      x1[k] = x[cells + e];
+      const float a0 = (om2 * eyv - s) * x0[k], a1 = (om2 * exv - s) * x1[k];
+      const float b0 = -(bb[(i * YY + jp) * MM + k] - b) * eyv;
+      const float b1 = (bb[(ip * YY + j) * MM + k] - b) * exv;
+      y0[k] = (a0 + b0) + (c - cc[(im * YY + j) * MM + k]);
+      y1[k] = (a1 + b1) + (c - cc[(i * YY + jm) * MM + k]);
+      y[e] = y0[k];
+      y[cells + e] = y1[k];
+      acc[k] += (double)x0[k] * y0[k] + (double)x1[k] * y1[k];
+      acc[MM + k] += (double)y0[k] * y0[k] + (double)y1[k] * y1[k];
+      acc[2 * MM + k] += (double)x0[k] * x0[k] + (double)x1[k] * x1[k];
+    }
+    if constexpr (P > 3 * MM) {
+      add_gram<MM>(acc, 3 * MM, y0);
+      add_gram<MM>(acc, 3 * MM, y1);
+    }
   }
-  __syncthreads();
+}
+
+// The stencil's second pass with all its sums into L.sum (Rayleigh sums, then
+// the Gram matrix of y), in one pass or two.
+template <int MM>
+__device__ __forceinline__ void op_pass2_sums(const float* __restrict__ x, float* __restrict__ y,
+                              const float* __restrict__ bc, const float* __restrict__ eps,
+                              int XX, int YY, float om2, float s, int n, ModeLds<MM>& L) {
+  constexpr int NQ = ModeLds<MM>::NQ, NPAIR = ModeLds<MM>::NPAIR;
+  if constexpr (kFuseGram<MM>) {
+    double acc[NQ + NPAIR];
+#pragma unroll
+    for (int q = 0; q < NQ + NPAIR; ++q) acc[q] = 0.0;
+    op_pass2<MM>(x, y, bc, eps, XX, YY, om2, s, acc);
+    block_sum<MM>(acc, L, L.sum);
+  } else {
+    {
+      double acc[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+      op_pass2<MM>(x, y, bc, eps, XX, YY, om2, s, acc);
+      block_sum<MM>(acc, L, L.sum);
+    }
+    double acc[NPAIR];
+#pragma unroll
+    for (int q = 0; q < NPAIR; ++q) acc[q] = 0.0;
+    for (int r = threadIdx.x; r < n; r += kModeThreads) {
+      float v[MM];
+#pragma unroll
+      for (int k = 0; k < MM; ++k) v[k] = y[r * MM + k];
+      add_gram<MM>(acc, 0, v);
+    }
+    block_sum<MM>(acc, L, L.sum + NQ);
+  }
 }
 
 template <int MM>
 __global__ __launch_bounds__(kModeThreads) void k_mode_subspace(ModeArgs a) {
   __shared__ ModeLds<MM> L;
+  constexpr int NQ = ModeLds<MM>::NQ, NPAIR = ModeLds<MM>::NPAIR;
   const int f = blockIdx.x;
   const int n = 2 * a.XX * a.YY;  // rows of the (n, MM) matrices
   const long long off = (long long)f * n * MM;
@@ -200,40 +252,55 @@ __global__ __launch_bounds__(kModeThreads) void k_mode_subspace(ModeArgs a) {
   const float om = a.omega[f];
   const float om2 = om * om;
   const float s = a.shift[f];
-  apply_op<MM>(x, y, bc, a.eps, a.XX, a.YY, om2, s);
-  constexpr int NQ = ModeLds<MM>::NQ;
+  // y = A x0 and its Gram matrix; then R^-1 for the first QR.
+  op_pass1<MM>(x, bc, a.eps, a.XX, a.YY);
+  op_pass2_sums<MM>(x, y, bc, a.eps, a.XX, a.YY, om2, s, n, L);
+  if (threadIdx.x == 0) chol_inv<MM>(L.sum + NQ, L);
+  __syncthreads();
   int it = 0;
   bool go = true;  // max err > tol; err starts at +inf
   while (it < a.max_iters && go) {
-    gram_chol<MM>(y, n, L);
-    apply_ri<MM>(y, x, n, L);
-    gram_chol<MM>(x, n, L);
-    apply_ri<MM>(x, x, n, L);
-    apply_op<MM>(x, y, bc, a.eps, a.XX, a.YY, om2, s);
-    double axy[MM], ayy[MM], axx[MM];
+    // CholeskyQR2 of y: x = y R1^-1 (with the Gram matrix of that x), then
+    // x = x R2^-1.
+    {
+      double acc[NPAIR];
 #pragma unroll
-    for (int k = 0; k < MM; ++k) axy[k] = ayy[k] = axx[k] = 0.0;
-    for (int r = threadIdx.x; r < n; r += kModeThreads) {
+      for (int q = 0; q < NPAIR; ++q) acc[q] = 0.0;
+      for (int r = threadIdx.x; r < n; r += kModeThreads) {
+        float v[MM];
 #pragma unroll
-      for (int k = 0; k < MM; ++k) {
-        const double xv = x[r * MM + k], yv = y[r * MM + k];
-        axy[k] += xv * yv;
-        ayy[k] += yv * yv;
-        axx[k] += xv * xv;
+        for (int k = 0; k < MM; ++k) v[k] = y[r * MM + k];
+        row_ri<MM>(v, L);
+#pragma unroll
+        for (int k = 0; k < MM; ++k) x[r * MM + k] = v[k];
+        if constexpr (MM <= 8) add_gram<MM>(acc, 0, v);
       }
-    }
+      if constexpr (MM > 8) {
+        // Above 8 modes the Gram sum takes a pass of its own (registers).
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += kModeThreads) {
+          float v[MM];
 #pragma unroll
-    for (int qc = 0; qc < NQ; qc += kChunk) {
-      double acc[kChunk];
-#pragma unroll
-      for (int p = 0; p < kChunk; ++p) {
-        const int q = qc + p;
-        acc[p] = q < MM ? axy[q < MM ? q : 0]
-                 : q < 2 * MM ? ayy[q < 2 * MM ? q - MM : 0]
-                 : q < NQ ? axx[q < NQ ? q - 2 * MM : 0] : 0.0;
+          for (int k = 0; k < MM; ++k) v[k] = x[r * MM + k];
+          add_gram<MM>(acc, 0, v);
+        }
       }
-      block_sum<MM>(acc, min(kChunk, NQ - qc), L, L.sum + qc);
+      block_sum<MM>(acc, L, L.sum + NQ);
+      if (threadIdx.x == 0) chol_inv<MM>(L.sum + NQ, L);
+      __syncthreads();
+      for (int r = threadIdx.x; r < n; r += kModeThreads) {
+        float v[MM];
+#pragma unroll
+        for (int k = 0; k < MM; ++k) v[k] = x[r * MM + k];
+        row_ri<MM>(v, L);
+#pragma unroll
+        for (int k = 0; k < MM; ++k) x[r * MM + k] = v[k];
+      }
+      __syncthreads();
     }
+    // y = A x with the Rayleigh sums and the Gram matrix of the new y.
+    op_pass1<MM>(x, bc, a.eps, a.XX, a.YY);
+    op_pass2_sums<MM>(x, y, bc, a.eps, a.XX, a.YY, om2, s, n, L);
     if (threadIdx.x == 0) {
       double emax = 0.0;
       bool nan = false;
@@ -249,11 +316,12 @@ __global__ __launch_bounds__(kModeThreads) void k_mode_subspace(ModeArgs a) {
       }
       // pjz continues while max(err) > tol, which a NaN ends.
       L.go = !nan && emax > (double)a.tol;
+      // R^-1 of the new y for the next iteration's first QR step.
+      chol_inv<MM>(L.sum + NQ, L);
     }
     __syncthreads();
     go = L.go;
     ++it;
-    __syncthreads();
   }
   if (threadIdx.x == 0) {
     a.iters[f] = it;
