@@ -117,6 +117,36 @@ def test_native_512_short_run_matches_oracle():
     assert err <= TOL
 
 
+def test_beyond_32_bit_offsets_matches_reference_kernel():
+  """1024 x 800 x 128 = 105M cells: E and H of one level span more than 2^31
+  bytes, past the two-step kernel's 32-bit buffer offsets, so the default
+  path runs the one-step kernel with 64-bit offsets.  A source plane near the
+  high end of x puts the non-zero fields at the largest offsets; the result
+  must equal the cell-by-cell reference kernel's (independent indexing)."""
+  X, Y, Z, tt = 1024, 800, 128, 8
+  rng = np.random.default_rng(21)
+  dev = torch.device("cuda", 0)
+  eps = torch.ones((3, X, Y, Z), dtype=torch.float32, device=dev)
+  sf = torch.as_tensor(rng.standard_normal((2, 1, Y, Z)).astype(np.float32), device=dev)
+  wave = torch.as_tensor(np.stack([np.sin(0.3 * np.arange(tt)), np.cos(0.3 * np.arange(tt))],
+                                  -1).astype(np.float32), device=dev)
+  kw = dict(epsilon=eps, dt=0.5, source_field=sf, source_waveform=wave, source_position=X - 12,
+            absorption_mask=torch.zeros((3, X, Y), device=dev),
+            pml_kappa=torch.ones((Z, 2), device=dev), pml_sigma=torch.zeros((Z, 2), device=dev),
+            pml_alpha=torch.zeros((Z, 2), device=dev), pml_widths=(0, 0),
+            output_steps=(tt - 1, tt, 1), use_reduced_precision=False, offset=(0, 0, 0))
+  got = F.fdtdz(**dict(kw, launch_params=None))
+  ref = F.fdtdz(**dict(kw, launch_params=dict(kernel="reference")))
+  torch.cuda.synchronize()
+  near = got[:, :, X - 24:]
+  assert float(torch.linalg.vector_norm(near)) > 0
+  err = float(torch.linalg.vector_norm(got - ref) / torch.linalg.vector_norm(ref))
+  print(f"105M cells: rel L2 default vs reference kernel {err:.2e}", flush=True)
+  assert err <= 1e-6
+  del got, ref
+  torch.cuda.empty_cache()
+
+
 def test_time_shift_invariance_is_exact():
   kw = random_problem(0, tt=100, seed=3)
   shift = 18
