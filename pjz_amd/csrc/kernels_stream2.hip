@@ -305,16 +305,16 @@ struct Lds2 {
 // (kOff) and its store is dropped, so every plane issues the same stores and
 // the wait for the next plane's prefetched loads does not also wait for them.
 struct Out2 {
-  rsrc_t eh, psi;
-  unsigned N4, H4, P4;
+  rsrc_t e, h, psi;  // E and H of the new level: one resource each
+  unsigned N4, P4;
 };
 
 __device__ inline Out2 make_out2(const Stream2Args& a) {
   Out2 o;
   o.N4 = 4u * (unsigned)a.N;
-  o.H4 = 4u * (unsigned)a.hoff;
   o.P4 = 4u * (unsigned)a.P;
-  o.eh = make_rsrc(a.Eout, o.H4 + 3 * o.N4);
+  o.e = make_rsrc(a.Eout, 3 * o.N4);
+  o.h = make_rsrc(a.Eout + a.hoff, 3 * o.N4);
   o.psi = make_rsrc(a.psiOut, 4 * o.P4);
   return o;
 }
@@ -462,8 +462,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
     const unsigned ob = st && lane_ok ? (unsigned)((((x - 1) * a.Y + y) * Z + zb) * 4) : kOff;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      bstore<CPL>(en2[k], O.eh, ob, k * O.N4);
-      bstore<CPL>(hn2[k], O.eh, ob, O.H4 + k * O.N4);
+      bstore<CPL>(en2[k], O.e, ob, k * O.N4);
+      bstore<CPL>(hn2[k], O.h, ob, k * O.N4);
     }
     const int pb = ((x - 1) * a.Y + y) * a.NP;
     if (CPL == 2 && a.pair) {
@@ -626,9 +626,13 @@ static void launch2(Stream2Args& a, hipStream_t s) {
 }
 
 // Whether the two-step kernel handles this problem (vector loads needed).
-// Buffer resources address at most 2^31 bytes: E and H of one level (24 B a
-// cell plus alignment) must fit, i.e. up to about 89M cells.
-static bool fits_rsrc(const Geom& g) { return g.N * 24 + 1024 <= (1LL << 31); }
+// Its store resources address at most 2^31 bytes: the three components of E
+// (or of H) of one level, 12 B a cell, must fit, i.e. up to about 178M cells
+// (the 32-bit element offsets of its loads reach 6 N, within 2^31 there), and
+// so must the four psi components of one set (16 B per PML cell).
+static bool fits_rsrc(const Geom& g) {
+  return g.N * 12 + 1024 <= (1LL << 31) && 16LL * g.X * g.Y * g.NP + 1024 <= (1LL << 31);
+}
 
 bool stream2_supported(const Geom& g) {
   return fits_rsrc(g) && g.Z <= 256 &&
