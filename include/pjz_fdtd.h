@@ -176,9 +176,12 @@ PJZ_FDTD_API int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsil
  *   w, err   (ww, mm)             Rayleigh quotients <x_k, A x_k> and residuals
  *                                 |A x_k - w_k x_k|, in basis order (unsorted)
  *   iters    (ww,)                iterations run per frequency
- * Each frequency iterates until max_iters or until every residual is <= tol
- * (pjz tests max(err) > tol before each iteration, over all frequencies at
- * once).  1 <= mm <= 10.  Workspace from pjz_fdtd_mode_workspace_bytes.
+ * Iteration runs until max_iters or until residuals are <= tol.  pjz tests
+ * max(err) > tol before each iteration, over all frequencies at once, and
+ * large sections do the same (one cooperative launch, several workgroups per
+ * frequency).  Small sections (xx * yy * mm < 16384) give each frequency one
+ * workgroup, and each frequency stops once its own residuals are <= tol.
+ * 1 <= mm <= 10.  Workspace from pjz_fdtd_mode_workspace_bytes.
  */
 PJZ_FDTD_API int pjz_fdtd_mode_workspace_bytes(int32_t ww, int32_t xx, int32_t yy, int32_t mm,
                                                size_t* bytes);
