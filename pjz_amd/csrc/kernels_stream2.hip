@@ -79,48 +79,62 @@ struct ZCoef {
   int pidx[CPL];
 };
 
-// Adds -J(step) to the curl terms of cell (x, y, zb + c).
+// Adds -J(step) to the curl terms of cells (x, y, zb + c).  As in
+// add_source, the two source components land in fixed slots on every branch
+// (a lane's z cell is picked by select, not by index) and are subtracted with
+// selects on the uniform axis, so the curl terms stay in registers.
 template <int CPL>
 __device__ inline void add_source2(const Stream2Args& a, int step, int x, int y, int zb,
                                    float (&cx)[CPL], float (&cy)[CPL], float (&cz)[CPL]) {
-  if (a.saxis == 0) {
-    if (x != a.spos && x != a.spos - 1) return;
-    const float wv = a.swave[2 * step + (x == a.spos ? 0 : 1)];
+  float ju[CPL], jv[CPL];  // J of (y, z), (x, z) or (x, y) for axis 0, 1, 2
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      if (zb + c < a.Z) {
-        const int so = y * a.Z + zb + c;
-        cy[c] -= a.sfield[so] * wv;
-        cz[c] -= a.sfield[a.Y * a.Z + so] * wv;
+  for (int c = 0; c < CPL; ++c) ju[c] = jv[c] = 0.0f;
+  if (a.saxis == 0) {
+    if (x == a.spos || x == a.spos - 1) {
+      const float wv = a.swave[2 * step + (x == a.spos ? 0 : 1)];
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        if (zb + c < a.Z) {
+          const int so = y * a.Z + zb + c;
+          ju[c] = a.sfield[so] * wv;
+          jv[c] = a.sfield[a.Y * a.Z + so] * wv;
+        }
       }
     }
   } else if (a.saxis == 1) {
-    if (y != a.spos && y != a.spos - 1) return;
-    const float wv = a.swave[2 * step + (y == a.spos ? 0 : 1)];
+    if (y == a.spos || y == a.spos - 1) {
+      const float wv = a.swave[2 * step + (y == a.spos ? 0 : 1)];
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      if (zb + c < a.Z) {
-        const int so = x * a.Z + zb + c;
-        cx[c] -= a.sfield[so] * wv;
-        cz[c] -= a.sfield[a.X * a.Z + so] * wv;
+      for (int c = 0; c < CPL; ++c) {
+        if (zb + c < a.Z) {
+          const int so = x * a.Z + zb + c;
+          ju[c] = a.sfield[so] * wv;
+          jv[c] = a.sfield[a.X * a.Z + so] * wv;
+        }
       }
     }
   } else {
-    const int c = a.spos - zb;
-    if (c < 0 || c >= CPL) return;
-    const int xy = x * a.Y + y, XY = a.X * a.Y;
-    float jx = a.sfield[xy] * a.swave[2 * step];
-    float jy = a.sfield[XY + xy] * a.swave[2 * step];
-    if (a.squad > 1) {
-      jx += a.sfield[2 * XY + xy] * a.swave[2 * step + 1];
-      jy += a.sfield[3 * XY + xy] * a.swave[2 * step + 1];
-    }
-#pragma unroll
-    for (int k = 0; k < CPL; ++k)
-      if (k == c) {
-        cx[k] -= jx;
-        cy[k] -= jy;
+    const int c0 = a.spos - zb;
+    if (c0 >= 0 && c0 < CPL) {
+      const int xy = x * a.Y + y, XY = a.X * a.Y;
+      float jx = a.sfield[xy] * a.swave[2 * step];
+      float jy = a.sfield[XY + xy] * a.swave[2 * step];
+      if (a.squad > 1) {
+        jx += a.sfield[2 * XY + xy] * a.swave[2 * step + 1];
+        jy += a.sfield[3 * XY + xy] * a.swave[2 * step + 1];
       }
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        ju[k] = k == c0 ? jx : 0.0f;
+        jv[k] = k == c0 ? jy : 0.0f;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    cx[c] -= a.saxis == 0 ? 0.0f : ju[c];
+    cy[c] -= a.saxis == 0 ? ju[c] : (a.saxis == 1 ? 0.0f : jv[c]);
+    cz[c] -= a.saxis == 2 ? 0.0f : jv[c];
   }
 }
 

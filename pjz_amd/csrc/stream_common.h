@@ -161,34 +161,42 @@ __device__ inline void z_prev(float (&o)[CPL], const float (&v)[CPL], int lane) 
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // Adds -J(n) of the source plane to the curl terms of cell (x, y, z).
+// The two source components land in fixed slots (ju, jv) on every branch and
+// are subtracted with selects on the (uniform) axis.  Subtracting into cx, cy
+// or cz on different branches lets the compiler merge the branches into a
+// store through a selected pointer: the curl terms then live in scratch
+// memory, and the wait for that scratch load also waits for every prefetched
+// load of the next plane.
 __device__ inline void add_source(const Src& src, const Geom& g, int n, int x, int y, int z,
                                   float& cx, float& cy, float& cz) {
-  if (z >= g.Z) return;
-  if (src.axis == 0) {
-    if (x == src.pos || x == src.pos - 1) {
-      const float wv = src.wave[2 * n + (x == src.pos ? 0 : 1)];
-      const long long so = (long long)y * g.Z + z;
-      cy -= src.field[so] * wv;
-      cz -= src.field[(long long)g.Y * g.Z + so] * wv;
+  float ju = 0.0f, jv = 0.0f;  // J of (y, z), (x, z) or (x, y) for axis 0, 1, 2
+  if (z < g.Z) {
+    if (src.axis == 0) {
+      if (x == src.pos || x == src.pos - 1) {
+        const float wv = src.wave[2 * n + (x == src.pos ? 0 : 1)];
+        const long long so = (long long)y * g.Z + z;
+        ju = src.field[so] * wv;
+        jv = src.field[(long long)g.Y * g.Z + so] * wv;
+      }
+    } else if (src.axis == 1) {
+      if (y == src.pos || y == src.pos - 1) {
+        const float wv = src.wave[2 * n + (y == src.pos ? 0 : 1)];
+        const long long so = (long long)x * g.Z + z;
+        ju = src.field[so] * wv;
+        jv = src.field[(long long)g.X * g.Z + so] * wv;
+      }
+    } else if (z == src.pos) {
+      const long long xy = (long long)x * g.Y + y, XY = (long long)g.X * g.Y;
+      for (int q = 0; q < src.quad; ++q) {
+        const float wv = src.wave[2 * n + q];
+        ju += src.field[(2 * q + 0) * XY + xy] * wv;
+        jv += src.field[(2 * q + 1) * XY + xy] * wv;
+      }
     }
-  } else if (src.axis == 1) {
-    if (y == src.pos || y == src.pos - 1) {
-      const float wv = src.wave[2 * n + (y == src.pos ? 0 : 1)];
-      const long long so = (long long)x * g.Z + z;
-      cx -= src.field[so] * wv;
-      cz -= src.field[(long long)g.X * g.Z + so] * wv;
-    }
-  } else if (z == src.pos) {
-    const long long xy = (long long)x * g.Y + y, XY = (long long)g.X * g.Y;
-    float jx = 0.0f, jy = 0.0f;
-    for (int q = 0; q < src.quad; ++q) {
-      const float wv = src.wave[2 * n + q];
-      jx += src.field[(2 * q + 0) * XY + xy] * wv;
-      jy += src.field[(2 * q + 1) * XY + xy] * wv;
-    }
-    cx -= jx;
-    cy -= jy;
   }
+  cx -= src.axis == 0 ? 0.0f : ju;
+  cy -= src.axis == 0 ? ju : (src.axis == 1 ? 0.0f : jv);
+  cz -= src.axis == 2 ? 0.0f : jv;
 }
 
 }  // namespace pjz
