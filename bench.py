@@ -426,10 +426,41 @@ def main():
   traffic = line["roofline"]["traffic"]
   if traffic:
     line["roofline"]["traffic_frac_of_copy"] = round(traffic / avg_launch_s / 1e9 / ceiling, 4)
+  if args.kernel == "auto" and not args.time_steps:
+    line["one_step_kernel"] = one_step_roofline(builder, dev, bytes_per_launch, cells)
   if not args.no_cpu_baseline:
     np_kw = builder(steps=args.time_steps) if args.time_steps else builder()
     line["cpu_baseline"] = cpu_baseline(np_kw)
   print(json.dumps(line), flush=True)
+
+
+def one_step_roofline(builder, dev, bytes_per_launch, cells, tsteps=1000):
+  """The one-step kernel (k_stream, one HBM pass per FDTD step) on the same
+  domain for `tsteps` steps: its roofline fraction beside the two-step
+  kernel's, which does more cell-updates per second at a lower fraction."""
+  import numpy as np
+  import torch
+  from pjz_amd import fdtdz, workloads
+
+  kw = builder(steps=tsteps)
+  for k, v in list(kw.items()):
+    if isinstance(v, np.ndarray):
+      kw[k] = torch.as_tensor(v).to(dev)
+  kw["launch_params"] = dict(kernel="stream")
+  n = workloads.time_steps(kw)
+  stream = torch.cuda.current_stream(dev)
+  fdtdz(**kw)
+  torch.cuda.synchronize()
+  start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  start.record(stream)
+  fdtdz(**kw)
+  end.record(stream)
+  torch.cuda.synchronize()
+  launch_s = start.elapsed_time(end) / 1e3 / n
+  achieved = bytes_per_launch / launch_s / 1e9
+  return {"kernel": "stream", "time_steps": int(n), "avg_launch_us": round(launch_s * 1e6, 2),
+          "gcell_updates_per_s": round(cells / launch_s / 1e9, 3),
+          "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4)}
 
 
 if __name__ == "__main__":
