@@ -74,6 +74,8 @@ typedef struct pjz_fdtd_problem {
   int32_t flags;         /* bit 0: use_reduced_precision (fp32 storage kept)     */
   int32_t kernel;        /* PJZ_FDTD_KERNEL_*                                    */
   int32_t tile_x, tile_y;/* launch overrides for the stream kernel, 0 = auto     */
+                         /* (batches: tile_y 8 or 13 on every zz <= 64 sim    */
+                         /* fixes that class's rows per tile)                  */
   int32_t waves_per_simd;/* occupancy hint for the stream kernel, 0 = auto      */
   int32_t ckpt_every;    /* > 0: forward saves a checkpoint every ckpt_every     */
                          /* steps so pjz_fdtd_backward can run on the workspace  */

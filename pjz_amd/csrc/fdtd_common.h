@@ -189,6 +189,7 @@ struct Batch2Plan {
   hipGraph_t graph;        // captured chunk, built on first use
   hipGraphExec_t exec;
   int count[2], base[2], total[2];  // per cells-per-lane class: sims, first sim, tiles
+  int wy[2];                        // per class: rows per tile
   char* meta;
   size_t args_bytes, rare_bytes, ts_bytes;
 };

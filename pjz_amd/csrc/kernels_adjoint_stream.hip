@@ -451,10 +451,20 @@ hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, const int
   for (int c = 0; c < 2; ++c) {
     plan.base[c] = off;
     plan.count[c] = (int)cls[c].size();
+    // x chunks from the same greedy-queue model as the forward batch, on the
+    // one workgroup per CU this kernel's launch uses.
+    std::vector<int> xs, ntys;
+    for (int i : cls[c]) {
+      xs.push_back(items[i].g.X);
+      ntys.push_back((items[i].g.Y + rows_per_tile(items[i].g) - 1) / rows_per_tile(items[i].g));
+    }
+    const int cLX = xs.empty() ? kAdjBatchLX
+                               : batch_chunk_x(xs.data(), ntys.data(), (int)xs.size(), plan.cus,
+                                               kAdjBatchLX);
     int t = 0;
     for (size_t k = 0; k < cls[c].size(); ++k) {
       const BatchItem& it = items[cls[c][k]];
-      const int LX = std::min(kAdjBatchLX, it.g.X);
+      const int LX = std::min(cLX, it.g.X);
       for (int v = 0; v < 2; ++v)
         hargs[2 * (off + k) + v] =
             adj_stream_args(it.g, *it.p, it.mask, it.ws, *it.L, it.capture, v, LX);

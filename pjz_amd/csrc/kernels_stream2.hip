@@ -792,7 +792,8 @@ bool batch2_supported(const Geom& g) {
 // Rows per batch tile by cells-per-lane class: zz <= 64 (CPL = 1) takes 13
 // rows (16 waves at ~120 registers, one workgroup filling a CU), zz <= 128
 // takes 8 (11 waves at 168 registers).
-static constexpr int kBatchWY1 = 13;
+static constexpr int kBatchWY1 = 13;       // cells per lane 1: rows per tile
+static constexpr int kBatchWY1Small = 8;   // ... when too few tiles fill the device
 static constexpr int kBatchWY2 = 8;
 static constexpr int kBatchLX = 128;
 
@@ -820,29 +821,40 @@ static int batch2_resident() {
 // on `slots` workgroups for each candidate LX and keep the longest chunk
 // within 3% of the shortest makespan (longer chunks re-read fewer ghost
 // planes, which the model does not charge).
+static long long batch_makespan(const int* xs, const int* nty, int n, int slots, int lx) {
+  std::priority_queue<long long, std::vector<long long>, std::greater<long long>> free_at;
+  for (int w = 0; w < slots; ++w) free_at.push(0);
+  long long span = 0;
+  for (int i = 0; i < n; ++i) {
+    const int ntx = (xs[i] + lx - 1) / lx;
+    for (int tx = 0; tx < ntx; ++tx) {
+      const long long d = std::min(lx, xs[i] - tx * lx) + 2;
+      for (int ty = 0; ty < nty[i]; ++ty) {
+        const long long t = free_at.top() + d;
+        free_at.pop();
+        free_at.push(t);
+        span = std::max(span, t);
+      }
+    }
+  }
+  return span;
+}
+
 int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx) {
-  static const int cands[] = {128, 96, 64, 48, 32};
+  // Down to 8 planes, the single-simulation minimum: a batch of one or a few
+  // small simulations needs short chunks to put a tile on every workgroup.
+  static const int cands[] = {128, 96, 64, 48, 32, 24, 16, 12, 8};
   std::vector<long long> cost;
   std::vector<int> lxs;
   for (int lx : cands) {
     if (lx > max_lx) continue;
-    std::priority_queue<long long, std::vector<long long>, std::greater<long long>> free_at;
-    for (int w = 0; w < slots; ++w) free_at.push(0);
-    long long span = 0;
-    for (int i = 0; i < n; ++i) {
-      const int ntx = (xs[i] + lx - 1) / lx;
-      for (int tx = 0; tx < ntx; ++tx) {
-        const long long d = std::min(lx, xs[i] - tx * lx) + 2;
-        for (int ty = 0; ty < nty[i]; ++ty) {
-          const long long t = free_at.top() + d;
-          free_at.pop();
-          free_at.push(t);
-          span = std::max(span, t);
-        }
-      }
-    }
-    cost.push_back(span);
+    cost.push_back(batch_makespan(xs, nty, n, slots, lx));
     lxs.push_back(lx);
+    // Eight or more rounds of tiles already keep every workgroup busy to
+    // within an eighth of a round: shorter chunks only add ghost planes.
+    long long tiles = 0;
+    for (int i = 0; i < n; ++i) tiles += (long long)nty[i] * ((xs[i] + lx - 1) / lx);
+    if (tiles >= 8LL * slots) break;
   }
   if (lxs.empty()) return max_lx;
   const long long best = *std::min_element(cost.begin(), cost.end());
@@ -892,15 +904,43 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
   for (int c = 0; c < 2; ++c) {
     plan.base[c] = off;
     plan.count[c] = (int)cls[c].size();
-    const int WY = c == 0 ? kBatchWY1 : kBatchWY2;
-    std::vector<int> xs, ntys;
-    for (int i : cls[c]) {
-      xs.push_back(items[i].g.X);
-      ntys.push_back((items[i].g.Y + WY - 1) / WY);
+    // Row tiles: 8 at two cells per lane; at one cell per lane 13 (16 waves,
+    // fewer ghost rows per owned row) unless 13-row tiles cannot give every
+    // workgroup one even at the shortest x chunk (8 planes), where 8-row
+    // tiles (11 waves) finish sooner.  Measured on 128x128x64 simulations:
+    // one alone runs 11% faster with 8-row tiles, two or more 3-26% slower.
+    std::vector<int> xs;
+    for (int i : cls[c]) xs.push_back(items[i].g.X);
+    auto chunk = [&](int wy, int resident) {
+      std::vector<int> ntys;
+      for (int i : cls[c]) ntys.push_back((items[i].g.Y + wy - 1) / wy);
+      return xs.empty() ? kBatchLX
+                        : batch_chunk_x(xs.data(), ntys.data(), (int)xs.size(),
+                                        plan.cus * resident, kBatchLX);
+    };
+    int WY = c == 0 ? kBatchWY1 : kBatchWY2;
+    if (c == 0) {
+      // The caller may fix the rows per tile: tile_y = 8 or 13 on every
+      // simulation of the class.
+      int forced = 0;
+      for (int i : cls[c]) {
+        const int ty = items[i].p->tile_y;
+        const bool ok = ty == kBatchWY1 || ty == kBatchWY1Small;
+        forced = ok && (forced == 0 || forced == ty) ? ty : -1;
+        if (forced < 0) break;
+      }
+      long long tiles13 = 0;
+      for (int i : cls[c])
+        tiles13 += (long long)((items[i].g.Y + kBatchWY1 - 1) / kBatchWY1) *
+                   ((items[i].g.X + 7) / 8);
+      const bool small = forced > 0 ? forced == kBatchWY1Small
+                                    : tiles13 < (long long)plan.cus * batch2_resident<1, kBatchWY1>();
+      if (small) WY = kBatchWY1Small;
     }
-    const int slots = plan.cus * (c == 0 ? batch2_resident<1, kBatchWY1>()
-                                         : batch2_resident<2, kBatchWY2>());
-    const int cLX = batch_chunk_x(xs.data(), ntys.data(), (int)xs.size(), slots, kBatchLX);
+    const int cLX = c == 1                       ? chunk(kBatchWY2, batch2_resident<2, kBatchWY2>())
+                    : WY == kBatchWY1Small ? chunk(WY, batch2_resident<1, kBatchWY1Small>())
+                                           : chunk(WY, batch2_resident<1, kBatchWY1>());
+    plan.wy[c] = WY;
     int t = 0;
     for (size_t k = 0; k < cls[c].size(); ++k) {
       const BatchItem& it = items[cls[c][k]];
@@ -935,7 +975,12 @@ static void launch_pair(const Batch2Plan& plan, hipStream_t s, int* ctr, const i
     if (nc == 0) continue;
     const Stream2Args* a = dargs + 2 * plan.base[c];
     const int* ts = dts + plan.base[c] + c;
-    if (c == 0) {
+    if (c == 0 && plan.wy[0] == kBatchWY1Small) {
+      const int grid = std::min(plan.total[c], plan.cus * batch2_resident<1, kBatchWY1Small>());
+      hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY1Small, true>), dim3(grid),
+                         dim3(64 * (kBatchWY1Small + 3)), 0, s, a, ts, nc, plan.total[c],
+                         ctr + c, step_base, step_off);
+    } else if (c == 0) {
       const int grid = std::min(plan.total[c], plan.cus * batch2_resident<1, kBatchWY1>());
       hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY1, true>), dim3(grid),
                          dim3(64 * (kBatchWY1 + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,

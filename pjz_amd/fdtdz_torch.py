@@ -354,7 +354,10 @@ def fdtdz_batch(calls):
   order of ``calls``.  When any ``epsilon`` requires grad, the batch is one
   ``torch.autograd.Function`` whose backward runs the batched adjoint
   (``pjz_fdtd_batch_backward``) for every simulation at once.  Every call needs
-  zz <= 64 or an even zz <= 128.
+  zz <= 64 or an even zz <= 128.  Tiles are chosen for the whole batch (x
+  chunks and, at zz <= 64, 8 or 13 rows per tile, from a model of the work
+  queue); ``launch_params`` ``tile_y`` = 8 or 13 on every zz <= 64 call fixes
+  those rows, and ``graph_steps`` replays chunks of step pairs as hipGraphs.
   """
   if len(calls) == 0:
     return []

@@ -39,12 +39,15 @@ CASES = [(1, 24, 18, 40, 70, 0), (2, 37, 21, 64, 61, 1), (3, 20, 29, 96, 70, 2),
          (4, 140, 17, 128, 44, 0), (5, 19, 30, 24, 53, 2)]
 
 
-def test_uneven_batch_matches_oracle_and_single_runs():
-  calls = [sim(*c) for c in CASES]
+# tile_y 0: rows per tile from the queue model (8 rows for batches this
+# small); 13: the 16-wave tiles large batches of zz <= 64 simulations get.
+@pytest.mark.parametrize("tile_y", [0, 13])
+def test_uneven_batch_matches_oracle_and_single_runs(tile_y):
+  calls = [dict(sim(*c), launch_params=dict(tile_y=tile_y)) for c in CASES]
   outs = F.fdtdz_batch(calls)
   torch.cuda.synchronize()
   for case, kw, out in zip(CASES, calls, outs):
-    print(f"case {case}", flush=True)
+    print(f"case {case} tile_y {tile_y}", flush=True)
     got = out.cpu().numpy()
     ref = C.fdtdz(**kw)
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
@@ -58,8 +61,10 @@ def test_uneven_batch_matches_oracle_and_single_runs():
       assert np.linalg.norm(got - single) <= 1e-6 * np.linalg.norm(single)
 
 
-def test_batch64_builder_small():
-  calls = workloads.batch64(steps=40, n=6, sizes=(32, 48), heights=(64, 128))
+@pytest.mark.parametrize("tile_y", [0, 8, 13])
+def test_batch64_builder_small(tile_y):
+  calls = [dict(kw, launch_params=dict(tile_y=tile_y))
+           for kw in workloads.batch64(steps=40, n=6, sizes=(32, 48), heights=(64, 128))]
   outs = F.fdtdz_batch(calls)
   for kw, out in zip(calls, outs):
     single = F.fdtdz(**dict(kw, launch_params=dict(kernel="stream2")))
@@ -90,11 +95,11 @@ GRAD_CASES = [(11, 24, 18, 40, 70, 0), (12, 30, 21, 64, 61, 1), (13, 20, 23, 96,
               (14, 19, 30, 24, 53, 2)]
 
 
-@pytest.mark.parametrize("ckpt", [8, 22])
-def test_batch_gradient_matches_single_and_oracle(ckpt):
+@pytest.mark.parametrize("ckpt,tile_y", [(8, 0), (22, 0), (22, 13)])
+def test_batch_gradient_matches_single_and_oracle(ckpt, tile_y):
   cases = [grad_case(*c) for c in GRAD_CASES]
   eps = [torch.tensor(kw["epsilon"], device="cuda", requires_grad=True) for kw, _ in cases]
-  calls = [dict(kw, epsilon=e, launch_params=dict(ckpt_every=ckpt))
+  calls = [dict(kw, epsilon=e, launch_params=dict(ckpt_every=ckpt, tile_y=tile_y))
            for (kw, _), e in zip(cases, eps)]
   outs = F.fdtdz_batch(calls)
   loss = sum((o * torch.as_tensor(g, device="cuda")).sum() for o, (_, g) in zip(outs, cases))
