@@ -12,7 +12,10 @@ so the oracle is pinned by checks a plausible mistake would fail:
   - kappa = inf decouples z slices (2-D cavity closed form per slice);
   - first-step closed form E^1 = -cb J(0) (source placement and sign);
   - CPML reflection below a bound (sign of the psi recursion);
-  - edge-replicated epsilon equals explicit padding;  linearity.
+  - edge-replicated epsilon equals explicit padding;  linearity;
+  - reciprocity (symmetric discrete Green's function: source placement on
+    the E samples, curl directions, per-cell epsilon and absorber);
+  - time-shift invariance (no step-dependent coefficient).
 """
 
 import math
@@ -272,3 +275,76 @@ def test_cpml_reflection_bound():
   err_bare = np.linalg.norm(bare - ref) / np.linalg.norm(ref)
   assert err_bare > 0.3          # the walls do reflect within the window/time
   assert err < 0.02 * err_bare   # the CPML removes nearly all of it
+
+
+def _point_response(eps, mask, src, comp_src, rec, comp_rec, wave, dbwd=None, monkeypatch=None):
+  """E_comp_rec at `rec` over every step, driven by a unit current in
+  component comp_src (x or y) at `src`, through a z-plane source."""
+  X, Y, Z = eps.shape[1:]
+  sf = np.zeros((2, 2, X, Y, 1))
+  sf[0, comp_src, src[0], src[1], 0] = 1.0
+  tt = wave.shape[0]
+  out = S.fdtdz(eps, 0.5, sf, wave, src[2], mask, np.ones((Z, 2)), np.zeros((Z, 2)),
+                np.zeros((Z, 2)), (0, 0), (0, tt, 1))
+  return out[:, comp_rec, rec[0], rec[1], rec[2]]
+
+
+def _reciprocity_pair():
+  rng = np.random.default_rng(11)
+  X, Y, Z = 9, 8, 10
+  eps = 1 + 4 * rng.random((3, X, Y, Z))      # anisotropic, diagonal: reciprocal
+  mask = 0.08 * rng.random((3, X, Y))        # conductivity: reciprocal loss
+  t = np.arange(60)
+  wave = np.zeros((60, 2))
+  wave[:, 0] = np.sin(0.4 * t) * np.exp(-((t - 15) / 6.0) ** 2)
+  A, B = (2, 3, 4), (6, 5, 7)
+  return eps, mask, wave, A, B
+
+
+def test_reciprocity():
+  """Yee's scheme in E is (eps d_t^2 + dt^2 C^T C) E = -dt d_t J with a
+  symmetric spatial operator, and the absorber adds a diagonal term: the
+  discrete Green's function is symmetric.  So Ey at B driven by Jx at A
+  equals Ex at A driven by Jy at B, step by step (no CPML: kappa = 1,
+  sigma = 0).  A wrong index or direction in one curl term, a source placed
+  off its E sample, or epsilon / absorber taken from a neighbouring cell on
+  one side breaks it."""
+  eps, mask, wave, A, B = _reciprocity_pair()
+  ab = _point_response(eps, mask, A, 0, B, 1, wave)
+  ba = _point_response(eps, mask, B, 1, A, 0, wave)
+  assert np.abs(ab).max() > 1e-4            # the signal has arrived
+  np.testing.assert_allclose(ab, ba, rtol=0, atol=1e-12 * np.abs(ab).max())
+
+
+def test_reciprocity_fails_for_planted_shift(monkeypatch):
+  """The reciprocity check has teeth: a backward x difference taken one
+  cell off (no longer the transpose of the forward one) breaks it."""
+  eps, mask, wave, A, B = _reciprocity_pair()
+  real = S.dbwd
+
+  def bad(xp, f, axis):
+    out = real(xp, f, axis)
+    return np.roll(out, 1, axis=0) if axis == 0 else out
+  monkeypatch.setattr(S, "dbwd", bad)
+  ab = _point_response(eps, mask, A, 0, B, 1, wave)
+  ba = _point_response(eps, mask, B, 1, A, 0, wave)
+  assert np.abs(ab - ba).max() > 1e-3 * np.abs(ab).max()
+
+
+def test_time_shift_invariance():
+  """Delaying the waveform by k rows delays every snapshot by k steps, bit
+  for bit, with CPML, absorber and an x-plane source: no coefficient depends
+  on the step index."""
+  rng = np.random.default_rng(12)
+  X, Y, Z, tt, k = 10, 8, 12, 40, 7
+  eps = 1 + 3 * rng.random((3, X, Y, Z))
+  common = dict(dt=0.5, source_field=rng.standard_normal((2, 1, Y, Z)), source_position=4,
+                absorption_mask=H.absorption_mask(X, Y, 3, 2e-2),
+                pml_kappa=1 + rng.random((Z, 2)), pml_sigma=H.pml_sigma((3, 3), Z, 0.5, 3.0),
+                pml_alpha=np.full((Z, 2), 0.05), pml_widths=(3, 3))
+  wave = rng.standard_normal((tt, 2))
+  late = np.concatenate([np.zeros((k, 2)), wave])
+  a = S.fdtdz(eps, source_waveform=wave, output_steps=(10, tt, 3), **common)
+  b = S.fdtdz(eps, source_waveform=late, output_steps=(10 + k, tt + k, 3), **common)
+  assert np.abs(a).max() > 0
+  np.testing.assert_array_equal(a, b)
