@@ -117,3 +117,42 @@ def test_no_grad_path_allocates_no_checkpoints():
     e = torch.tensor(eps, device="cuda", requires_grad=True)
     out = F.fdtdz(e, **kw)
   assert not out.requires_grad
+
+
+def test_gradient_full_size_directional():
+  """The gradient benchmark's domain (256x256x128) over 800 steps, with its
+  overlap objective: the adjoint's directional derivative <dL/deps, v>
+  against central differences of the float32 forward, Richardson-extrapolated
+  from steps h and h/2 (the oracle cannot run this size in a test).  v raises
+  the permittivity of the waveguide core between the source and the monitor
+  (randomly modulated), so the objective's change is coherent and well above
+  float32 rounding."""
+  dev = torch.device("cuda", 0)
+  kw, mode, omegas = workloads.grad_256(steps=800)
+  eps0 = torch.as_tensor(kw.pop("epsilon"), device=dev)
+  for k, val in list(kw.items()):
+    if isinstance(val, np.ndarray):
+      kw[k] = torch.as_tensor(val, device=dev)
+  kw["launch_params"] = dict(ckpt_every=64)
+
+  def loss(e):
+    return workloads.overlap_objective(F.fdtdz(e, **kw), mode, omegas, kw["output_steps"])
+
+  print("grad_256 800 steps: directional derivative", flush=True)
+  e = eps0.clone().requires_grad_(True)
+  loss(e).backward()
+  gen = torch.Generator(device=dev).manual_seed(5)
+  v = torch.zeros_like(eps0)
+  core = (slice(None), slice(60, 200), slice(104, 152), slice(48, 64))
+  v[core] = 0.5 + torch.rand(v[core].shape, generator=gen, device=dev)
+  analytic = float((e.grad * v).sum())
+  h = 1e-2
+  with torch.no_grad():
+    def central(step):
+      return (float(loss(eps0 + step * v)) - float(loss(eps0 - step * v))) / (2 * step)
+    fd1, fd2 = central(h), central(h / 2)
+  fd = (4 * fd2 - fd1) / 3
+  print(f"adjoint {analytic:.6e}, central differences {fd1:.6e} (h), {fd2:.6e} (h/2), "
+        f"extrapolated {fd:.6e}", flush=True)
+  assert abs(fd) > 0
+  assert abs(analytic - fd) <= 5e-3 * abs(fd)
