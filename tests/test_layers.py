@@ -1,18 +1,24 @@
 """Permittivity render (pjz_amd.render / epsilon), torch port of pjz's
 ``src/pjz/_epsilon.py``.
 
-Every case and expected value is the reference's own (tests/test_layers.py):
+Every case and expected value is the reference's own (tests/test_layers.py),
+read from tests/golden/reference_vectors.json:
 threshold positions for each component and axis, sub-pixel interpolation as an
 interface sweeps a cell, z interfaces, and the corner cases whose values are
 closed-form harmonic/arithmetic mixes.  Run in float64 on the CPU and, for a
 subset, in float32 on the GPU.
 """
 
+import json
+import pathlib
+
 import numpy as np
 import pytest
 import torch
 
 from pjz_amd import _epsilon as E
+
+GOLDEN = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
 
 
 def zgrid(zz, dtype=torch.float64):
@@ -26,17 +32,8 @@ def two_layers(xx, yy, m, thresh, vals=(1, 3), dtype=torch.float64):
   return layer
 
 
-SIMPLE = [
-    ((5, 0, -0.5), (0, 1, 0, 0), 1.0), ((5, 0, -0.5), (0, 2, 0, 0), 1.5),
-    ((5, 0, -0.5), (0, 3, 0, 0), 3.0), ((4, 0, -0.5), (1, 1, 0, 0), 1.0),
-    ((4, 0, -0.5), (1, 2, 0, 0), 2.0), ((4, 0, -0.5), (1, 3, 0, 0), 3.0),
-    ((0, 5, -0.5), (1, 0, 1, 0), 1.0), ((0, 5, -0.5), (1, 0, 2, 0), 1.5),
-    ((0, 5, -0.5), (1, 0, 3, 0), 3.0), ((0, 4, -0.5), (2, 0, 1, 0), 1.0),
-    ((0, 4, -0.5), (2, 0, 2, 0), 2.0), ((0, 4, -0.5), (2, 0, 3, 0), 3.0),
-    ((0, 0, +2.5), (2, 0, 0, 1), 1.0), ((0, 0, +2.5), (2, 0, 0, 2), 1.5),
-    ((0, 0, +2.5), (2, 0, 0, 3), 3.0), ((0, 0, +2.0), (1, 0, 0, 1), 1.0),
-    ((0, 0, +2.0), (1, 0, 0, 2), 2.0), ((0, 0, +2.0), (1, 0, 0, 3), 3.0),
-]
+SIMPLE = [(tuple(c["thresh"]), tuple(c["index"]), c["expected"])
+          for c in GOLDEN["render_simple"]["cases"]]
 
 
 @pytest.mark.parametrize("thresh,index,expected", SIMPLE)
@@ -47,16 +44,8 @@ def test_simple(thresh, index, expected):
   assert float(out[index]) == pytest.approx(expected)
 
 
-X_THRESHOLD = [
-    (1, 3, 0, 0, 1.0), (1, 3, 0, 1, 1.5), (1, 3, 0, 2, 3.0), (2, 6, 0, 0, 1.0),
-    (2, 6, 0, 1, 1.5), (2, 6, 0, 2, 3.0), (3, 9, 0, 0, 1.0), (3, 9, 0, 1, 1.5),
-    (3, 9, 0, 2, 3.0), (3, 6, 0, 1, 3.0), (3, 7, 0, 1, 9 / 4), (3, 8, 0, 1, 9 / 5),
-    (3, 9, 0, 1, 1.5), (3, 10, 0, 1, 9 / 7), (3, 11, 0, 1, 9 / 8), (3, 12, 0, 1, 1.0),
-    (1, 4, 1, 1, 1.0), (1, 4, 1, 2, 2.0), (1, 4, 1, 3, 3.0), (2, 8, 1, 1, 1.0),
-    (2, 8, 1, 2, 2.0), (2, 8, 1, 3, 3.0), (3, 12, 1, 1, 1.0), (3, 12, 1, 2, 2.0),
-    (3, 12, 1, 3, 3.0), (3, 9, 1, 2, 3.0), (3, 10, 1, 2, 8 / 3), (3, 11, 1, 2, 7 / 3),
-    (3, 12, 1, 2, 2.0), (3, 13, 1, 2, 5 / 3), (3, 14, 1, 2, 4 / 3), (3, 15, 1, 2, 1.0),
-]
+X_THRESHOLD = [(c["m"], c["thresh"], c["component"], c["index"], c["expected"])
+               for c in GOLDEN["render_x_threshold"]["cases"]]
 
 
 @pytest.mark.parametrize("m,thresh,component,index,expected", X_THRESHOLD)
@@ -68,11 +57,8 @@ def test_x_threshold(m, thresh, component, index, expected):
   assert float(out[component, index, 0, 0]) == pytest.approx(expected)
 
 
-Z_THRESHOLD = [
-    (0.00, 2, 0, 3.0), (0.25, 2, 0, 2.0), (0.50, 2, 0, 1.5), (0.75, 2, 0, 1.2),
-    (1.00, 2, 0, 1.0), (-0.50, 0, 0, 3.0), (-0.25, 0, 0, 2.5), (+0.00, 0, 0, 2.0),
-    (+0.25, 0, 0, 1.5), (+0.50, 0, 0, 1.0),
-]
+Z_THRESHOLD = [(c["thresh"], c["component"], c["index"], c["expected"])
+               for c in GOLDEN["render_z_threshold"]["cases"]]
 
 
 @pytest.mark.parametrize("thresh,component,index,expected", Z_THRESHOLD)
@@ -83,28 +69,10 @@ def test_z_threshold(thresh, component, index, expected):
   assert float(out[component, 0, 0, index]) == pytest.approx(expected)
 
 
-HXY = 1 / (0.5 * (5 / 6) + 0.5 * (2 / 3))
-HXYZ = 1 / ((1 / 3) * (11 / 12) + (2 / 3) * (4 / 5))
-CORNER = [
-    ((0, 0, -0.5), (0, 0, 1, 0), 3.0), ((1, 0, -0.5), (0, 0, 1, 0), 1.5),
-    ((2, 0, -0.5), (0, 0, 1, 0), 1.0), ((0, 2, -0.5), (0, 0, 1, 0), 2.0),
-    ((0, 3, -0.5), (0, 0, 1, 0), 1.0), ((0, 0, +0.0), (0, 0, 1, 0), 2.0),
-    ((0, 0, +0.5), (0, 0, 1, 0), 1.0), ((1, 2, -0.5), (0, 0, 1, 0), HXY),
-    ((0, 2, +0.0), (0, 0, 1, 0), 1.5), ((1, 0, +0.0), (0, 0, 1, 0), HXY),
-    ((1, 2, +0.0), (0, 0, 1, 0), HXYZ),
-    ((0, 0, -0.5), (1, 1, 0, 0), 3.0), ((2, 0, -0.5), (1, 1, 0, 0), 2.0),
-    ((3, 0, -0.5), (1, 1, 0, 0), 1.0), ((0, 1, -0.5), (1, 1, 0, 0), 1.5),
-    ((0, 2, -0.5), (1, 1, 0, 0), 1.0), ((0, 0, +0.0), (1, 1, 0, 0), 2.0),
-    ((0, 0, +0.5), (1, 1, 0, 0), 1.0), ((2, 1, -0.5), (1, 1, 0, 0), HXY),
-    ((0, 1, +0.0), (1, 1, 0, 0), HXY), ((2, 0, +0.0), (1, 1, 0, 0), 1.5),
-    ((2, 1, +0.0), (1, 1, 0, 0), HXYZ),
-    ((0, 0, 0.0), (2, 1, 1, 0), 3.0), ((2, 0, 0.0), (2, 1, 1, 0), 2.0),
-    ((3, 0, 0.0), (2, 1, 1, 0), 1.0), ((0, 2, 0.0), (2, 1, 1, 0), 2.0),
-    ((0, 3, 0.0), (2, 1, 1, 0), 1.0), ((0, 0, 0.5), (2, 1, 1, 0), 1.5),
-    ((0, 0, 1.0), (2, 1, 1, 0), 1.0), ((2, 2, 0.0), (2, 1, 1, 0), 1.5),
-    ((0, 2, 0.5), (2, 1, 1, 0), HXY), ((2, 0, 0.5), (2, 1, 1, 0), HXY),
-    ((2, 2, 0.5), (2, 1, 1, 0), HXYZ),
-]
+# Corner mixes, harmonic and arithmetic: 1/(0.5 (5/6) + 0.5 (2/3)) and
+# 1/((1/3)(11/12) + (2/3)(4/5)) among them.
+CORNER = [(tuple(c["thresh"]), tuple(c["index"]), c["expected"])
+          for c in GOLDEN["render_corner"]["cases"]]
 
 
 @pytest.mark.parametrize("thresh,index,expected", CORNER)

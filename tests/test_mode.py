@@ -9,13 +9,18 @@ modes of the strip do not reach tol in either; 20000 puts all four betas
 within 5e-4 of the golden values), to keep the CPU suite short.
 """
 
+import json
+import pathlib
+
 import numpy as np
 import pytest
 import torch
 
 from pjz_amd import _mode as M
 
-GOLDEN_BETAS = (0.36388508, 0.18891069, 0.15406249, 0.13549446)
+GOLDEN_BETAS = tuple(json.loads(
+    (pathlib.Path(__file__).parent / "golden" / "reference_vectors.json").read_text()
+)["mode_betas"]["betas"])
 
 
 def strip(prop_axis, uu=30, vv=20):

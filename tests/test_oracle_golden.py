@@ -2,35 +2,37 @@
 
 Expected values are the literal vectors and properties of the reference tests
 tests/test_boundaries.py:8-41, tests/test_waveforms.py:8-14 and
-tests/test_frequencies.py:8-38, re-targeted at the oracle's names.
+tests/test_frequencies.py:8-38, re-targeted at the oracle's names; the vectors
+are read from tests/golden/reference_vectors.json.
 """
+
+import json
+import pathlib
 
 import numpy as np
 import pytest
 
 from oracle import helpers as H
 
+GOLDEN = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
+
 
 def test_absorption_mask_rows():  # tests/test_boundaries.py:8-24
-  m = H.absorption_mask(xx=10, yy=10, width=3, smoothness=4)
-  np.testing.assert_array_equal(m[0, :, 5], [36, 16, 4, 0, 0, 0, 0, 4, 16, 36])
-  np.testing.assert_array_equal(m[0, 5, :], [25, 9, 1, 0, 0, 0, 1, 9, 25, 49])
-  np.testing.assert_array_equal(m[1, :, 5], [25, 9, 1, 0, 0, 0, 1, 9, 25, 49])
-  np.testing.assert_array_equal(m[1, 5, :], [36, 16, 4, 0, 0, 0, 0, 4, 16, 36])
-  np.testing.assert_array_equal(m[2, :, 5], [25, 9, 1, 0, 0, 0, 1, 9, 25, 49])
-  np.testing.assert_array_equal(m[2, 5, :], [25, 9, 1, 0, 0, 0, 1, 9, 25, 49])
+  g = GOLDEN["absorption_mask"]
+  m = H.absorption_mask(**g["inputs"])
+  for row in g["rows"]:
+    got = m[row["component"], :, row["y"]] if row["x"] is None else m[row["component"], row["x"], :]
+    np.testing.assert_array_equal(got, row["values"])
 
 
-PML_TABLE = [[8.0000000e+01, 4.6894531e+01], [2.5312500e+01, 1.2207031e+01],
-             [5.0000000e+00, 1.5820312e+00], [3.1250000e-01, 1.9531250e-02],
-             [0.0, 0.0], [0.0, 0.0],
-             [1.9531250e-02, 3.1250000e-01], [1.5820312e+00, 5.0000000e+00],
-             [1.2207031e+01, 2.5312500e+01], [4.6894531e+01, 8.0000000e+01]]
+PML_TABLE = GOLDEN["pml_sigma"]["table"]
 
 
 def test_pml_sigma_table():  # tests/test_boundaries.py:27-41
+  i = GOLDEN["pml_sigma"]["inputs"]
   np.testing.assert_array_almost_equal(
-      H.pml_sigma(pml_widths=(4, 4), zz_=10, ln_R=16.0, m=4.0), PML_TABLE)
+      H.pml_sigma(pml_widths=tuple(i["pml_widths"]), zz_=i["zz"], ln_R=i["ln_R"], m=i["m"]),
+      PML_TABLE)
 
 
 def test_pml_sigma_zero_width():
