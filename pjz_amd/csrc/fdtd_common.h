@@ -140,6 +140,7 @@ hipError_t launch_copy_stream(void* dst, const void* src, size_t n16, hipStream_
 // x chunk length for a batch class: minimises rounds x planes per tile over
 // candidate lengths <= max_lx (xs, nty: each simulation's X and row tiles).
 int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx);
+int grid_chunk_x(int X, int nty, int slots, int ghost);
 
 // Waveguide mode solver (kernels_mode.hip): ww frequencies of a (xx, yy)
 // cross-section, 1 <= mm <= kModeMaxModes modes, one workgroup per frequency.

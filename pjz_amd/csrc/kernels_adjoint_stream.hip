@@ -394,8 +394,7 @@ static AdjStreamArgs adj_stream_args(const Geom& g, const pjz_fdtd_problem& p, c
   const int WY = rows_per_tile(g);
   a.nty = (g.Y + WY - 1) / WY;
   if (LX <= 0) {
-    const int ntx = std::max(1, (device_cus() + a.nty - 1) / a.nty);
-    LX = std::max(8, (g.X + ntx - 1) / ntx);
+    LX = grid_chunk_x(g.X, a.nty, device_cus(), 1);
   }
   a.LX = LX;
   a.ntiles = a.nty * ((g.X + a.LX - 1) / a.LX);

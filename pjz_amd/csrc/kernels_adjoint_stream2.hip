@@ -557,8 +557,7 @@ static Adj2Args adj2_args(const Geom& g, const pjz_fdtd_problem& p, const float*
   if (LX <= 0) {
     const int slots =
         device_cus2() * (g.Z <= 64 ? adj2_resident_single<1>() : adj2_resident_single<2>());
-    const int ntx = std::max(1, (slots + a.nty - 1) / a.nty);
-    LX = std::max(8, (g.X + ntx - 1) / ntx);
+    LX = grid_chunk_x(g.X, a.nty, slots, 2);
   }
   a.LX = LX;
   a.ntiles = a.nty * ((g.X + a.LX - 1) / a.LX);

@@ -355,9 +355,7 @@ hipError_t run_stream(const Geom& g, const pjz_fdtd_problem& p, const float* mas
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int nty = (g.Y + wy - 1) / wy;
-    const int ntx = std::max(1, (cus + nty - 1) / nty);
-    a.LX = std::max(8, (g.X + ntx - 1) / ntx);
+    a.LX = grid_chunk_x(g.X, (g.Y + wy - 1) / wy, cus, 1);
   }
   float* E0 = (float*)(ws + L.E0);
   float* H0 = (float*)(ws + L.H0);

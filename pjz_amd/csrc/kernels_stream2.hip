@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <functional>
 #include <queue>
+#include <utility>
 #include <vector>
 
 #include "stream_common.h"
@@ -737,10 +738,8 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
   if (cpl == 1 && p.tile_y == 13) WY = 13;
   int LX = p.tile_x;
   if (LX <= 0) {
-    // x chunks sized so the grid has about one workgroup per CU.
-    const int nty = (g.Y + WY - 1) / WY;
-    const int ntx = std::max(1, (device_cus() + nty - 1) / nty);
-    LX = std::max(8, (g.X + ntx - 1) / ntx);
+    // x chunks sized so the grid fills the CUs in whole rounds.
+    LX = grid_chunk_x(g.X, (g.Y + WY - 1) / WY, device_cus(), 2);
   }
   Stream2Args a = make_args2(g, p, mask, src, win, ws, L, WY, LX, 0);
   Stream2Rare* rare = (Stream2Rare*)(ws + L.rare);
@@ -862,6 +861,31 @@ int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx) {
     if (cost[c] * 100 <= best * 103) return lxs[c];
   return lxs[0];
 }
+// x chunk length of a single simulation's grid (nty row tiles per chunk,
+// `ghost` recomputed planes per chunk).  The device runs `slots` workgroups
+// at a time, so a grid just above a multiple of `slots` ends with a nearly
+// empty extra round of tiles.  A chunk length costs rounds x (planes + ghost
+// planes + 2 for a tile's start-up); keep the longest chunk within 3% of the
+// cheapest.  Chunks are at least 8 planes (or all of X).
+int grid_chunk_x(int X, int nty, int slots, int ghost) {
+  slots = std::max(slots, 1);
+  std::vector<std::pair<long long, int>> c;  // (cost, chunk), longest first
+  for (int ntx = 1;; ++ntx) {
+    const int lx = (X + ntx - 1) / ntx;
+    if (ntx > 1 && lx < 8) break;
+    if (c.empty() || c.back().second != lx) {
+      const long long tiles = (long long)nty * ((X + lx - 1) / lx);
+      c.push_back({(tiles + slots - 1) / slots * (long long)(lx + ghost + 2), lx});
+    }
+    if (lx <= 8) break;
+  }
+  long long best = c.front().first;
+  for (const auto& e : c) best = std::min(best, e.first);
+  for (const auto& e : c)
+    if (e.first * 100 <= best * 103) return e.second;
+  return c.front().second;
+}
+
 static constexpr int kMaxChunkPairs = 64;  // step pairs per captured graph chunk
 
 size_t batch2_meta_bytes(const BatchItem* items, int n) {
