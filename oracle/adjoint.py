@@ -5,7 +5,7 @@ of ``fdtd_spec``.  One forward step is
     psiH' = bH psiH + cH Dzf E            (PML cells)
     H'    = H - dt (Cf E)                 (z terms ikH Dzf E + psiH')
     psiE' = bE psiE + cE Dzb H'
-    E'    = ca E + cb K,  K = Cb H' - J   (z terms ikE Dzb H' + psiE')
+    E'    = ca E + cb K,  K = Cb H' + J   (z terms ikE Dzb H' + psiE')
 and the reverse step, with lK = cb lE', runs
     kernel 1 (H cells):  lH_tot = lH' + Cb^T lK + psiE terms;  u = lpsiH' +- dt lH_tot
     kernel 2 (E cells):  lE = ca lE' + dt Cf^T-gather(lH_tot) + psiH terms;

@@ -106,6 +106,6 @@ def run(eps_full, mask, dt, kappa, sigma, alpha, pml_widths, src_axis, src_pos,
             q = mask[cc, i, j] * dt / (2 * e)
             ca = (1 - q) / (1 + q)
             cb = (dt / e) / (1 + q)
-            En[cc, i, j, k] = ca * E[cc, i, j, k] + cb * (curl[cc] - J(cc, i, j, k, n))
+            En[cc, i, j, k] = ca * E[cc, i, j, k] + cb * (curl[cc] + J(cc, i, j, k, n))
     E = En
   return E, H

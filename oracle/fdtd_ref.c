@@ -118,17 +118,17 @@ int oracle_fdtd_run(int X, int Y, int Z, int lo, int hi, double dt, const double
           cur[2] = (Hy[c0] - hy_xm) - (Hx[c0] - hx_ym);
           if (src_axis == 0 && (i == src_pos || i == src_pos - 1)) {
             double w = wave[2 * n + (i == src_pos ? 0 : 1)];
-            cur[1] -= src_field[(long)j * Z + k] * w;
-            cur[2] -= src_field[(long)Y * Z + (long)j * Z + k] * w;
+            cur[1] += src_field[(long)j * Z + k] * w;
+            cur[2] += src_field[(long)Y * Z + (long)j * Z + k] * w;
           } else if (src_axis == 1 && (j == src_pos || j == src_pos - 1)) {
             double w = wave[2 * n + (j == src_pos ? 0 : 1)];
-            cur[0] -= src_field[(long)i * Z + k] * w;
-            cur[2] -= src_field[(long)X * Z + (long)i * Z + k] * w;
+            cur[0] += src_field[(long)i * Z + k] * w;
+            cur[2] += src_field[(long)X * Z + (long)i * Z + k] * w;
           } else if (src_axis == 2 && k == src_pos) {
             long xy = (long)i * Y + j, XY = (long)X * Y;
             for (int q = 0; q < src_quad; ++q) {
-              cur[0] -= src_field[(2 * q + 0) * XY + xy] * wave[2 * n + q];
-              cur[1] -= src_field[(2 * q + 1) * XY + xy] * wave[2 * n + q];
+              cur[0] += src_field[(2 * q + 0) * XY + xy] * wave[2 * n + q];
+              cur[1] += src_field[(2 * q + 1) * XY + xy] * wave[2 * n + q];
             }
           }
           for (int c = 0; c < 3; ++c) {

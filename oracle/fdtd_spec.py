@@ -14,7 +14,7 @@ Update for step n = 0, 1, ... (unit cells, mu = 1, c = 1):
                                            is  dz/kappa_H(k) + psi_H  with
                                            psi_H = b_H psi_H + c_H dz  (column 1
                                            of the pml arrays, z = k + 1/2)
-  E half step  E = ca*E + cb*(curl_b(H) - J(n))   backward differences, d/dz
+  E half step  E = ca*E + cb*(curl_b(H) + J(n))   backward differences, d/dz
                                            terms with column 0 (z = k)
 
   cb = (dt/eps) / (1 + q),  ca = (1 - q)/(1 + q),  q = sigma_abs*dt/(2 eps),
@@ -31,6 +31,17 @@ Update for step n = 0, 1, ... (unit cells, mu = 1, c = 1):
       J = sf[0] * w[n,0] + sf[1] * w[n,1].
   Output s in range(*output_steps) is E after the E half step of step s,
   over the eps window at ``offset``: out (nn, 3, xx, yy, ze).
+
+  The source enters with a plus sign (the field of a physical current -J).
+  pjz's scatter() gradient, sum Re(g f_i f_j / a_i) (_field.py:380-382,
+  393-398), is odd in the sign of the fields, while the S-values are not.
+  Under this update the reciprocity term is K dS_ij/deps with K close to
+  Omega ww eps_mode exp(-i omega dt / 2) > 0 (Omega = 2 sin(omega dt/2)/dt,
+  ww frequencies, eps_mode a modal average of eps), measured against finite
+  differences in tests/test_oracle_scatter.py; with -J it would be -K, a
+  gradient that points uphill.  pjz's optimisation loop
+  (examples/_utils.py:27-29) descends along that gradient, so the sign that
+  makes it a descent direction is taken.
 
 Pinning: brute-force per-cell loops (``brute.py``), discrete energy
 conservation, closed-form cavity-mode evolution, first-step source closed
@@ -213,14 +224,14 @@ class Simulation:
       out.append((axis, idx, plane * amp))
     return out
 
-  def _sub_source(self, f, comp, n):
+  def _add_source(self, f, comp, n):
     xp = self.xp
     for axis, idx, slab in self._j(comp, n):
       shp = list(f.shape)
       shp[axis] = 1
       slab = slab.reshape(shp) if hasattr(slab, "reshape") else slab
       f = _cat(xp, [_take(f, axis, slice(0, idx)),
-                    _take(f, axis, slice(idx, idx + 1)) - slab,
+                    _take(f, axis, slice(idx, idx + 1)) + slab,
                     _take(f, axis, slice(idx + 1, None))], axis)
     return f
 
@@ -253,9 +264,9 @@ class Simulation:
       tzHx = self._with_psi(ikE * dzHx, self.psiE[1])
     else:
       tzHy, tzHx = ikE * dzHy, ikE * dzHx
-    cx = self._sub_source(dbwd(xp, Hz, 1) - tzHy, 0, n)
-    cy = self._sub_source(tzHx - dbwd(xp, Hz, 0), 1, n)
-    cz = self._sub_source(dbwd(xp, Hy, 0) - dbwd(xp, Hx, 1), 2, n)
+    cx = self._add_source(dbwd(xp, Hz, 1) - tzHy, 0, n)
+    cy = self._add_source(tzHx - dbwd(xp, Hz, 0), 1, n)
+    cz = self._add_source(dbwd(xp, Hy, 0) - dbwd(xp, Hx, 1), 2, n)
     Ex = self.ca[0] * Ex + self.cb[0] * cx
     Ey = self.ca[1] * Ey + self.cb[1] * cy
     Ez = self.ca[2] * Ez + self.cb[2] * cz

@@ -34,7 +34,7 @@ def overlap(mode, beta, pos, is_fwd, output):  # _field.py:305-338
 
 
 def scatter(epsilon, omega, modes, betas, pos, is_fwd, p):
-  """(svals, grads) as _scatter_impl builds them (_field.py:346-384)."""
+  """(svals, grads, fields) as _scatter_impl builds them (_field.py:346-384)."""
   fields = [field_ref.field(epsilon, np.mean(m, axis=0), omega, q, p)
             for m, q in zip(modes, pos)]
   amps = [np.ones_like(b, dtype=np.complex128) if fwd is None else overlap(m, b, q, fwd, f)[:, 0]
@@ -43,7 +43,7 @@ def scatter(epsilon, omega, modes, betas, pos, is_fwd, p):
            for a, f in zip(amps, fields)]
   grads = [[fi * fj / a[:, None, None, None, None] for fj in fields]
            for a, fi in zip(amps, fields)]
-  return svals, grads
+  return svals, grads, fields
 
 
 def scatter_vjp(grads, g):  # _field.py:393-398

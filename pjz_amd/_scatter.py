@@ -6,8 +6,9 @@ planes next to each port split into forward/backward amplitudes, S-values
 normalised by the injected amplitude, and a custom backward pass that
 returns pjz's reciprocity gradient  sum_ij Re(g_ij * f_i * f_j / a_i)
 (``_field.py:380-382,393-398``) instead of differentiating through the
-simulations.  The per-port simulations run as one batch on the GPU
-(:func:`pjz_amd._field.fields`).
+simulations.  ``g_ij`` there is jax's cotangent dL/dRe - i dL/dIm; torch's is
+its conjugate, which the backward conjugates back.  The per-port
+simulations run as one batch on the GPU (:func:`pjz_amd._field.fields`).
 """
 
 from typing import Any, Sequence
@@ -82,10 +83,12 @@ class _Scatter(torch.autograd.Function):
     for i, (fi, ai) in enumerate(zip(fields, amps)):
       for j, fj in enumerate(fields):
         gradij = fi * fj / ai[:, None, None, None, None].to(fi.dtype)
-        term = torch.sum(torch.real(g[i, j][:, None, None, None, None].to(gradij.dtype) * gradij),
-                         dim=0)
+        # torch hands complex outputs the cotangent dL/dRe + i dL/dIm, the
+        # conjugate of the one jax passes to pjz's _scatter_bwd; conjugating it
+        # gives the same gradient as the reference for the same loss.
+        gij = torch.conj(g[i, j])[:, None, None, None, None].to(gradij.dtype)
+        term = torch.sum(torch.real(gij * gradij), dim=0)
         grad = term if grad is None else grad + term
-    ctx.fields = ctx.amps = None
     return (grad.to(device=ctx.eps_device, dtype=ctx.eps_dtype), None, None, None, None, None,
             None)
 

@@ -150,7 +150,12 @@ hipError_t autotune(const pjz::Geom& g, const pjz_fdtd_problem& p, const float* 
     cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 0, 0});
     cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 64, 0});
     if (cpl2) cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 0, 4});
-    if (g.Z <= 64) cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 0, 13});
+    // At zz <= 64 the default (tile_y 0) picks 13 or 8 rows by the CU count;
+    // both are timed explicitly.
+    if (g.Z <= 64) {
+      cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 0, 13});
+      cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 0, 8});
+    }
   }
   if (g.Z <= 256) {
     cands.push_back({PJZ_FDTD_KERNEL_STREAM, 0, 0});

@@ -176,16 +176,16 @@ __global__ void k_ref_e(float* __restrict__ E, const float* __restrict__ H,
     if (i == src.pos || i == src.pos - 1) {
       float w = src.wave[2 * n + (i == src.pos ? 0 : 1)];
       long long o = (long long)j * g.Z + k;
-      cy -= src.field[o] * w;
-      cz -= src.field[YZ + o] * w;
+      cy += src.field[o] * w;
+      cz += src.field[YZ + o] * w;
     }
   } else if (src.axis == 1) {
     if (j == src.pos || j == src.pos - 1) {
       float w = src.wave[2 * n + (j == src.pos ? 0 : 1)];
       long long XZ = (long long)g.X * g.Z;
       long long o = (long long)i * g.Z + k;
-      cx -= src.field[o] * w;
-      cz -= src.field[XZ + o] * w;
+      cx += src.field[o] * w;
+      cz += src.field[XZ + o] * w;
     }
   } else if (k == src.pos) {
     long long XY = (long long)g.X * g.Y;
@@ -195,8 +195,8 @@ __global__ void k_ref_e(float* __restrict__ E, const float* __restrict__ H,
       jx += src.field[(2 * q + 0) * XY + xy] * w;
       jy += src.field[(2 * q + 1) * XY + xy] * w;
     }
-    cx -= jx;
-    cy -= jy;
+    cx += jx;
+    cy += jy;
   }
   const float c3[3] = {cx, cy, cz};
   bool in_win = out_idx >= 0 && i >= win.x0 && i < win.x0 + win.ex && j >= win.y0 &&

@@ -80,9 +80,9 @@ struct ZCoef {
   int pidx[CPL];
 };
 
-// Adds -J(step) to the curl terms of cells (x, y, zb + c).  As in
+// Adds J(step) to the curl terms of cells (x, y, zb + c).  As in
 // add_source, the two source components land in fixed slots on every branch
-// (a lane's z cell is picked by select, not by index) and are subtracted with
+// (a lane's z cell is picked by select, not by index) and are added with
 // selects on the uniform axis, so the curl terms stay in registers.
 template <int CPL>
 __device__ inline void add_source2(const Stream2Args& a, int step, int x, int y, int zb,
@@ -133,9 +133,9 @@ __device__ inline void add_source2(const Stream2Args& a, int step, int x, int y,
   }
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    cx[c] -= a.saxis == 0 ? 0.0f : ju[c];
-    cy[c] -= a.saxis == 0 ? ju[c] : (a.saxis == 1 ? 0.0f : jv[c]);
-    cz[c] -= a.saxis == 2 ? 0.0f : jv[c];
+    cx[c] += a.saxis == 0 ? 0.0f : ju[c];
+    cy[c] += a.saxis == 0 ? ju[c] : (a.saxis == 1 ? 0.0f : jv[c]);
+    cz[c] += a.saxis == 2 ? 0.0f : jv[c];
   }
 }
 

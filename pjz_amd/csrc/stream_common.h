@@ -160,13 +160,13 @@ __device__ inline void z_prev(float (&o)[CPL], const float (&v)[CPL], int lane) 
 // Rows exchange data only through LDS, so LDS ordering is all that is needed.
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// Adds -J(n) of the source plane to the curl terms of cell (x, y, z).
-// The two source components land in fixed slots (ju, jv) on every branch and
-// are subtracted with selects on the (uniform) axis.  Subtracting into cx, cy
-// or cz on different branches lets the compiler merge the branches into a
-// store through a selected pointer: the curl terms then live in scratch
-// memory, and the wait for that scratch load also waits for every prefetched
-// load of the next plane.
+// Adds J(n) of the source plane to the curl terms of cell (x, y, z) (the
+// update is E = ca E + cb (curl H + J)).  The two source components land in
+// fixed slots (ju, jv) on every branch and are added with selects on the
+// (uniform) axis.  Adding into cx, cy or cz on different branches lets the
+// compiler merge the branches into a store through a selected pointer: the
+// curl terms then live in scratch memory, and the wait for that scratch load
+// also waits for every prefetched load of the next plane.
 __device__ inline void add_source(const Src& src, const Geom& g, int n, int x, int y, int z,
                                   float& cx, float& cy, float& cz) {
   float ju = 0.0f, jv = 0.0f;  // J of (y, z), (x, z) or (x, y) for axis 0, 1, 2
@@ -194,9 +194,9 @@ __device__ inline void add_source(const Src& src, const Geom& g, int n, int x, i
       }
     }
   }
-  cx -= src.axis == 0 ? 0.0f : ju;
-  cy -= src.axis == 0 ? ju : (src.axis == 1 ? 0.0f : jv);
-  cz -= src.axis == 2 ? 0.0f : jv;
+  cx += src.axis == 0 ? 0.0f : ju;
+  cy += src.axis == 0 ? ju : (src.axis == 1 ? 0.0f : jv);
+  cz += src.axis == 2 ? 0.0f : jv;
 }
 
 }  // namespace pjz

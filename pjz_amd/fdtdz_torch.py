@@ -181,7 +181,7 @@ def _forward(prob, dev, device):
 
 def _backward(prob, dev, device, ws, grad_out):
   grad_eps = torch.empty((3, prob.ex, prob.ey, prob.ez), dtype=torch.float32, device=device)
-  if ws is None:
+  if _lib.num_outputs(prob) == 0:  # no steps run: nothing depends on epsilon
     return grad_eps.zero_()
   g = grad_out.detach().to(device=device, dtype=torch.float32).contiguous()
   with torch.cuda.device(device):
@@ -195,19 +195,29 @@ def _backward(prob, dev, device, ws, grad_out):
 
 
 class FdtdzFunction(torch.autograd.Function):
-  """fdtdz() with a gradient for ``epsilon`` from the adjoint kernels."""
+  """fdtdz() with a gradient for ``epsilon`` from the adjoint kernels.
+
+  The checkpoint workspace is released after the first backward (it holds
+  several full field states).  A later backward through the same graph
+  (``retain_graph=True``) first reruns the forward simulation to rebuild it:
+  the same gradient, for one more forward pass.
+  """
 
   @staticmethod
   def forward(ctx, epsilon, prob, dev, device):
     out, ws = _forward(prob, dev, device)
     ctx.prob, ctx.dev, ctx.device, ctx.ws = prob, dev, device, ws
+    ctx.eps_device, ctx.eps_dtype = epsilon.device, epsilon.dtype
     return out
 
   @staticmethod
   def backward(ctx, grad_out):
-    grad = _backward(ctx.prob, ctx.dev, ctx.device, ctx.ws, grad_out)
+    ws = ctx.ws
     ctx.ws = None
-    return grad, None, None, None
+    if ws is None and _lib.num_outputs(ctx.prob) > 0:
+      ws = _forward(ctx.prob, ctx.dev, ctx.device)[1]
+    grad = _backward(ctx.prob, ctx.dev, ctx.device, ws, grad_out)
+    return grad.to(device=ctx.eps_device, dtype=ctx.eps_dtype), None, None, None
 
 
 def fdtdz(epsilon, dt, source_field, source_waveform, source_position,
@@ -319,22 +329,29 @@ def _batch_backward(probs, devs, device, ws, grad_outs):
 
 
 class FdtdzBatchFunction(torch.autograd.Function):
-  """A batch of fdtdz() calls with gradients for every ``epsilon``."""
+  """A batch of fdtdz() calls with gradients for every ``epsilon``.
+
+  As for :class:`FdtdzFunction`, the workspace is released after the first
+  backward and rebuilt by a rerun of the batched forward for a later one.
+  """
 
   @staticmethod
   def forward(ctx, probs, devs, device, *epsilons):
     outs, ws = _batch_forward(probs, devs, device)
     ctx.probs, ctx.devs, ctx.device, ctx.ws = probs, devs, device, ws
-    ctx.eps_devices = [e.device for e in epsilons]
+    ctx.eps_meta = [(e.device, e.dtype) for e in epsilons]
     return tuple(outs)
 
   @staticmethod
   def backward(ctx, *grad_outs):
-    grads = _batch_backward(ctx.probs, ctx.devs, ctx.device, ctx.ws, grad_outs)
+    ws = ctx.ws
     ctx.ws = None
+    if ws is None:
+      ws = _batch_forward(ctx.probs, ctx.devs, ctx.device)[1]
+    grads = _batch_backward(ctx.probs, ctx.devs, ctx.device, ws, grad_outs)
     return (None, None, None) + tuple(
-        g.to(d) if need else None
-        for g, d, need in zip(grads, ctx.eps_devices, ctx.needs_input_grad[3:]))
+        g.to(device=d, dtype=t) if need else None
+        for g, (d, t), need in zip(grads, ctx.eps_meta, ctx.needs_input_grad[3:]))
 
 
 def batch_ckpt_every(probs):

@@ -1,0 +1,77 @@
+"""Host logic of the autograd Functions around the kernels (no GPU).
+
+The launches are replaced by stand-ins that record their calls, so these tests
+check what the Functions do with the workspace and the gradient's placement:
+a second backward through one graph reruns the forward to rebuild the
+released checkpoints and returns the same gradient, and the gradient comes
+back on epsilon's device and in its dtype.
+"""
+
+import numpy as np
+import torch
+
+from pjz_amd import fdtdz_torch as F
+
+
+def _problem(steps=10):
+  return F.make_problem((3, 4, 4, 4), 0.5, (2, 1, 4, 4), 2, (3, 4, 4), (4, 2), (0, 0),
+                        (steps - 2, steps, 1), steps, False, None, (0, 0, 0))
+
+
+class _Recorder:
+
+  def __init__(self):
+    self.forwards = 0
+    self.backward_ws = []
+
+  def forward(self, prob, dev, device):
+    self.forwards += 1
+    out = torch.ones((F._lib.num_outputs(prob), 3, prob.ex, prob.ey, prob.ez))
+    return out, object()
+
+  def backward(self, prob, dev, device, ws, grad_out):
+    self.backward_ws.append(ws)
+    return grad_out.sum(dim=0).to(torch.float32) * 2.0
+
+  def batch_forward(self, probs, devs, device):
+    self.forwards += 1
+    outs = [torch.ones((F._lib.num_outputs(p), 3, p.ex, p.ey, p.ez)) for p in probs]
+    return outs, object()
+
+  def batch_backward(self, probs, devs, device, ws, grad_outs):
+    self.backward_ws.append(ws)
+    return [g.sum(dim=0).to(torch.float32) * 3.0 for g in grad_outs]
+
+
+def test_single_backward_twice_reruns_forward(monkeypatch):
+  rec = _Recorder()
+  monkeypatch.setattr(F, "_forward", rec.forward)
+  monkeypatch.setattr(F, "_backward", rec.backward)
+  eps = torch.ones((3, 4, 4, 4), dtype=torch.float64, requires_grad=True)
+  out = F.FdtdzFunction.apply(eps, _problem(), None, torch.device("cpu"))
+  w = torch.arange(out.numel(), dtype=torch.float32).reshape(out.shape)
+  loss = (out * w).sum()
+  g1, = torch.autograd.grad(loss, eps, retain_graph=True)
+  g2, = torch.autograd.grad(loss, eps)
+  assert rec.forwards == 2  # the second backward rebuilt the workspace
+  assert all(ws is not None for ws in rec.backward_ws)
+  assert g1.dtype == torch.float64 and g1.device == eps.device
+  np.testing.assert_array_equal(g1.numpy(), g2.numpy())
+  np.testing.assert_array_equal(g1.numpy(), (2.0 * w.sum(dim=0)).numpy())
+
+
+def test_batch_backward_twice_reruns_forward(monkeypatch):
+  rec = _Recorder()
+  monkeypatch.setattr(F, "_batch_forward", rec.batch_forward)
+  monkeypatch.setattr(F, "_batch_backward", rec.batch_backward)
+  probs = [_problem(), _problem(12)]
+  eps = [torch.ones((3, 4, 4, 4), dtype=torch.float64, requires_grad=True) for _ in probs]
+  outs = F.FdtdzBatchFunction.apply(probs, None, torch.device("cpu"), *eps)
+  loss = sum((o * (i + 1)).sum() for i, o in enumerate(outs))
+  g1 = torch.autograd.grad(loss, eps, retain_graph=True)
+  g2 = torch.autograd.grad(loss, eps)
+  assert rec.forwards == 2
+  assert all(ws is not None for ws in rec.backward_ws)
+  for a, b, e in zip(g1, g2, eps):
+    assert a.dtype == e.dtype and a.device == e.device
+    np.testing.assert_array_equal(a.numpy(), b.numpy())

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 import torch
 
+import cw_checks as CW
 from oracle import field_ref
 from pjz_amd import _field as Fd
 
@@ -98,3 +99,22 @@ def test_fields_batch_equals_field_per_source(gpu):
     err = float(torch.linalg.vector_norm(g - ref) / torch.linalg.vector_norm(ref))
     print(f"port at {p}: rel {err:.2e}", flush=True)
     assert err <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("axis,pos,cplx", [("x", 7, False), ("y", 6, False), ("z", 5, True)])
+def test_field_is_the_cw_steady_state(axis, pos, cplx, gpu):
+  """field() on the GPU (float32) satisfies the time-harmonic form of the
+  discrete update: zero residual off the source plane, mode / eps with pjz's
+  phase and 1/ww on it (derivation in tests/cw_checks.py)."""
+  shape = (16, 14, 12)
+  eps = 1.5 + 2.0 * np.random.default_rng(0).random((3,) + shape)
+  p = P._replace(tt=2500, absorption_padding=8)
+  omega = np.array([2 * np.pi / 21, 2 * np.pi / 19])
+  mode, J = CW.divergence_free_source(axis, shape, eps, pos, complex_=cplx)
+  got = Fd.field(torch.as_tensor(eps, dtype=torch.float32, device=gpu),
+                 torch.as_tensor(mode, device=gpu), omega, pos, p).cpu().numpy()
+  err = CW.cw_error(got.astype(np.complex128), eps.astype(np.float32).astype(np.float64), mode,
+                    J, omega, pos, axis, p)
+  print(f"field {axis}@{pos}: CW residual {err:.2e}", flush=True)
+  assert err <= 2e-3

@@ -10,7 +10,7 @@ so the oracle is pinned by checks a plausible mistake would fail:
   - per-step mode damping det(M) = exp(-sigma dt / eps) to third order
     (absorber coefficients, sign and factor 2);
   - kappa = inf decouples z slices (2-D cavity closed form per slice);
-  - first-step closed form E^1 = -cb J(0) (source placement and sign);
+  - first-step closed form E^1 = cb J(0) (source placement and sign);
   - CPML reflection below a bound (sign of the psi recursion);
   - edge-replicated epsilon equals explicit padding;  linearity;
   - reciprocity (symmetric discrete Green's function: source placement on
@@ -190,7 +190,7 @@ def test_kappa_inf_decouples_z_slices():
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
-def test_first_step_is_minus_cb_J(axis):
+def test_first_step_is_cb_J(axis):
   rng = np.random.default_rng(10 + axis)
   X, Y, Z, dt = 6, 7, 8, 0.5
   eps = 1 + rng.random((3, X, Y, Z))
@@ -212,7 +212,7 @@ def test_first_step_is_minus_cb_J(axis):
   else:
     for c in range(2):
       J[c, :, :, pos] = sf[0, c, :, :, 0] * wave[0, 0] + sf[1, c, :, :, 0] * wave[0, 1]
-  np.testing.assert_allclose(out[0], -cb * J, rtol=0, atol=1e-14)
+  np.testing.assert_allclose(out[0], cb * J, rtol=0, atol=1e-14)
 
 
 def test_edge_fill_equals_explicit_padding():

@@ -1,4 +1,5 @@
-"""scatter(): amplitude solve on CPU, S-values and reciprocity gradient on GPU."""
+"""scatter(): amplitude solve and cotangent convention on CPU, S-values and
+reciprocity gradient on GPU."""
 
 import numpy as np
 import pytest
@@ -38,6 +39,64 @@ def test_overlap_matches_oracle_on_cpu():
     np.testing.assert_allclose(got.numpy(), ref, rtol=1e-10, atol=1e-12)
 
 
+def _linear_scatter(monkeypatch, seed=0, n=2, ww=2, shape=(3, 4, 3, 2)):
+  """Replaces the simulations by S-values linear in epsilon whose exact
+  derivative is pjz's reciprocity term: s_ij = sum_k (f_i f_j / a_i)[k] eps[k]."""
+  rng = np.random.default_rng(seed)
+  cplx = lambda *s: torch.as_tensor(rng.standard_normal(s) + 1j * rng.standard_normal(s))
+  fields = [cplx(ww, *shape) for _ in range(n)]
+  amps = [cplx(ww) for _ in range(n)]
+  grads = [[fi * fj / ai[:, None, None, None, None] for fj in fields]
+           for fi, ai in zip(fields, amps)]
+
+  def fake_impl(epsilon, omega, modes, betas, pos, is_fwd, sim_params):
+    e = epsilon.to(torch.complex128)
+    svals = [[torch.sum(gij * e, dim=(1, 2, 3, 4)) for gij in row] for row in grads]
+    return svals, amps, fields
+
+  monkeypatch.setattr(Sc, "_impl", fake_impl)
+  return grads, shape
+
+
+def test_backward_cotangent_convention(monkeypatch):
+  # With S linear in eps and dS_ij/deps = f_i f_j / a_i exactly, pjz's backward
+  # (jax cotangent) is the true gradient; the torch Function must equal it and
+  # torch.autograd through the same linear map.
+  grads, shape = _linear_scatter(monkeypatch)
+  eps0 = np.random.default_rng(5).random(shape) + 1.0
+  e = torch.tensor(eps0, requires_grad=True)
+  svals = Sc.scatter(e, None, [None, None], [np.zeros(2), np.zeros(2)], (0, 0), (True, False),
+                     None)
+  loss = sum(torch.abs(svals[i][j]).pow(2).sum() + 0.3 * torch.angle(svals[i][j]).sum()
+             for i in range(2) for j in range(2))
+  loss.backward()
+  e2 = torch.tensor(eps0, requires_grad=True)
+  ref = [[torch.sum(gij * e2.to(torch.complex128), dim=(1, 2, 3, 4)) for gij in row]
+         for row in grads]
+  loss2 = sum(torch.abs(ref[i][j]).pow(2).sum() + 0.3 * torch.angle(ref[i][j]).sum()
+              for i in range(2) for j in range(2))
+  loss2.backward()
+  np.testing.assert_allclose(e.grad.numpy(), e2.grad.numpy(), rtol=1e-10, atol=1e-12)
+  # The oracle restatement of pjz's backward, fed jax's cotangent of
+  # L = sum |s|^2 + 0.3 arg(s): dL/dRe - i dL/dIm = 2 conj(s) - 0.3 i / s.
+  s = np.array([[r.detach().numpy() for r in row] for row in ref])
+  g_jax = 2 * np.conj(s) - 0.3j / s
+  got = scatter_ref.scatter_vjp([[g.numpy() for g in row] for row in grads], g_jax)
+  np.testing.assert_allclose(got, e2.grad.numpy(), rtol=1e-10, atol=1e-12)
+
+
+def test_backward_twice(monkeypatch):
+  # retain_graph: a second backward through scatter() gives the same gradient.
+  _, shape = _linear_scatter(monkeypatch, seed=3)
+  e = torch.tensor(np.ones(shape), requires_grad=True)
+  svals = Sc.scatter(e, None, [None, None], [np.zeros(2), np.zeros(2)], (0, 0), (True, False),
+                     None)
+  loss = torch.abs(svals[0][1]).pow(2).sum()
+  g1, = torch.autograd.grad(loss, e, retain_graph=True)
+  g2, = torch.autograd.grad(loss, e)
+  np.testing.assert_array_equal(g1.numpy(), g2.numpy())
+
+
 def ports(seed=0, xx=16, yy=12, zz=16):
   rng = np.random.default_rng(seed)
   eps = (1 + 3 * rng.random((3, xx, yy, zz))).astype(np.float32)
@@ -52,7 +111,7 @@ def ports(seed=0, xx=16, yy=12, zz=16):
 def test_scatter_and_gradient_match_oracle(gpu):
   eps, modes, betas, pos, fwd = ports()
   omega = np.array([2 * np.pi / 20.5, 2 * np.pi / 19.5])
-  ref_s, ref_grads = scatter_ref.scatter(eps, omega, modes, betas, pos, fwd, P)
+  ref_s, ref_grads, _ = scatter_ref.scatter(eps, omega, modes, betas, pos, fwd, P)
   e = torch.tensor(eps, device=gpu, requires_grad=True)
   svals = Sc.scatter(e, omega, [torch.as_tensor(m, device=gpu) for m in modes],
                      [torch.as_tensor(b) for b in betas], pos, fwd, P)
@@ -66,10 +125,33 @@ def test_scatter_and_gradient_match_oracle(gpu):
   loss = sum(torch.sum(torch.real(svals[i][j] * torch.as_tensor(np.conj(g[i, j]), device=gpu)))
              for i in range(2) for j in range(2))
   loss.backward()
-  # The custom backward returns sum Re(g_ij * f_i f_j / a_i) for the cotangent
-  # torch hands it; for this loss that cotangent is g itself.
-  ref_grad = scatter_ref.scatter_vjp(ref_grads, g)
+  # For loss = sum Re(s_ij conj(g_ij)) jax hands pjz's backward the cotangent
+  # dL/dRe - i dL/dIm = conj(g_ij) (torch's own cotangent is g_ij).
+  ref_grad = scatter_ref.scatter_vjp(ref_grads, np.conj(g))
   got_grad = e.grad.cpu().numpy()
   gerr = np.linalg.norm(got_grad - ref_grad) / np.linalg.norm(ref_grad)
   print(f"scatter grad rel err {gerr:.2e}")
   assert gerr <= 1e-3
+
+
+@pytest.mark.gpu
+def test_straight_guide_on_the_gpu(gpu):
+  """The product's scatter() on a straight guide (mode from the HIP mode
+  solver): |S21| = |S12| = 1, S12 = S21, small S11 (the source sheet's near
+  field), as the oracle shows in tests/test_oracle_scatter.py."""
+  from pjz_amd import _mode
+  xx, yy, ze = 40, 22, 20
+  eps = np.ones((3, xx, yy, ze), np.float32)
+  eps[:, :, 8:14, 7:13] = 4.0
+  omega = np.array([2 * np.pi / 21, 2 * np.pi / 19])
+  e = torch.as_tensor(eps, device=gpu)
+  beta, exc, _, _ = _mode.mode(e[:, :1], torch.as_tensor(omega, device=gpu), 1)
+  m, b = exc[..., 0], beta[:, 0]
+  p = P._replace(tt=4000, absorption_padding=30, absorption_coeff=1.1e-3)
+  S = Sc.scatter(e, omega, [m, m], [b, b], (8, xx - 9), (True, False), p)
+  S = np.array([[s.detach().cpu().numpy() for s in row] for row in S])
+  print("|S|", np.abs(S).round(4).tolist(), flush=True)
+  np.testing.assert_allclose(np.abs(S[0, 1]), 1.0, atol=0.03)
+  np.testing.assert_allclose(np.abs(S[1, 0]), 1.0, atol=0.03)
+  assert np.max(np.abs(S[1, 0] - S[0, 1])) <= 3e-3
+  assert np.max(np.abs(S[0, 0])) <= 0.1 and np.max(np.abs(S[1, 1])) <= 0.1
