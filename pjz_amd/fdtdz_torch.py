@@ -283,12 +283,56 @@ def _batch_problems(calls):
   return probs, devs, device
 
 
-def _batch_workspace(probs, device):
+def _batch_workspace_bytes(probs):
   lib = _lib.load()
   nbytes = ctypes.c_size_t(0)
   _lib.check(lib.pjz_fdtd_batch_workspace_bytes(probs, len(probs), ctypes.byref(nbytes)),
              "pjz_fdtd_batch_workspace_bytes")
-  return _Workspace(nbytes.value, device)
+  return nbytes.value
+
+
+def _batch_workspace(probs, device):
+  return _Workspace(_batch_workspace_bytes(probs), device)
+
+
+def _subset(probs, idx, ckpt_every=None):
+  """ctypes array of the problems ``idx`` (copies; ``ckpt_every`` if given)."""
+  sub = (_lib.Problem * len(idx))()
+  for j, i in enumerate(idx):
+    ctypes.memmove(ctypes.byref(sub[j]), ctypes.byref(probs[i]), ctypes.sizeof(_lib.Problem))
+    if ckpt_every is not None:
+      sub[j].ckpt_every = ckpt_every
+  return sub
+
+
+def plan_waves(probs, budget):
+  """Splits a differentiated batch into consecutive waves whose gradient
+  workspaces (checkpoints, per-step E changes of one segment, adjoint state)
+  each fit in ``budget`` bytes; a list of index lists."""
+  waves, cur = [], []
+  for i in range(len(probs)):
+    if cur and _batch_workspace_bytes(_subset(probs, cur + [i])) > budget:
+      waves.append(cur)
+      cur = []
+    if not cur and _batch_workspace_bytes(_subset(probs, [i])) > budget:
+      raise ValueError(f"batch item {i}: its gradient workspace of "
+                       f"{_batch_workspace_bytes(_subset(probs, [i])) / 2**30:.1f} GiB exceeds the "
+                       f"memory budget of {budget / 2**30:.1f} GiB")
+    cur.append(i)
+  if cur:
+    waves.append(cur)
+  return waves
+
+
+def default_memory_budget(probs, device):
+  """Device memory a differentiated batch may give its workspaces: 90% of
+  what is free (including torch's cached blocks) less room for the outputs
+  and for their cotangents, which exist alongside the workspace during the
+  backward."""
+  free, _ = torch.cuda.mem_get_info(device)
+  cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+  outs = sum(4 * _lib.num_outputs(p) * 3 * p.ex * p.ey * p.ez for p in probs)
+  return int(0.9 * (free + cached)) - 2 * outs
 
 
 def _ptrs(tensors):
@@ -331,14 +375,30 @@ def _batch_backward(probs, devs, device, ws, grad_outs):
 class FdtdzBatchFunction(torch.autograd.Function):
   """A batch of fdtdz() calls with gradients for every ``epsilon``.
 
-  As for :class:`FdtdzFunction`, the workspace is released after the first
-  backward and rebuilt by a rerun of the batched forward for a later one.
+  ``waves`` splits the batch into groups whose gradient workspaces fit in
+  device memory one at a time.  The forward advances every wave; only the
+  last keeps its checkpoints (the others run without them).  The backward
+  takes the waves last to first and reruns the forward of each wave but the
+  last with checkpoints before its adjoint: one extra forward pass for all
+  but the last wave, instead of every wave's checkpoints at once.  A later
+  backward through the same graph (``retain_graph=True``) reruns every
+  wave's forward.
   """
 
   @staticmethod
-  def forward(ctx, probs, devs, device, *epsilons):
-    outs, ws = _batch_forward(probs, devs, device)
-    ctx.probs, ctx.devs, ctx.device, ctx.ws = probs, devs, device, ws
+  def forward(ctx, probs, devs, device, waves, *epsilons):
+    outs = [None] * len(probs)
+    ws = None
+    for k, wave in enumerate(waves):
+      last = k == len(waves) - 1
+      sub = _subset(probs, wave, ckpt_every=None if last else 0)
+      wouts, wws = _batch_forward(sub, [devs[i] for i in wave], device)
+      for i, o in zip(wave, wouts):
+        outs[i] = o
+      if last:
+        ws = wws
+      del wws
+    ctx.probs, ctx.devs, ctx.device, ctx.waves, ctx.ws = probs, devs, device, waves, ws
     ctx.eps_meta = [(e.device, e.dtype) for e in epsilons]
     return tuple(outs)
 
@@ -346,12 +406,20 @@ class FdtdzBatchFunction(torch.autograd.Function):
   def backward(ctx, *grad_outs):
     ws = ctx.ws
     ctx.ws = None
-    if ws is None:
-      ws = _batch_forward(ctx.probs, ctx.devs, ctx.device)[1]
-    grads = _batch_backward(ctx.probs, ctx.devs, ctx.device, ws, grad_outs)
-    return (None, None, None) + tuple(
+    grads = [None] * len(ctx.probs)
+    for k in range(len(ctx.waves) - 1, -1, -1):
+      wave = ctx.waves[k]
+      sub = _subset(ctx.probs, wave)
+      devs = [ctx.devs[i] for i in wave]
+      if ws is None:
+        ws = _batch_forward(sub, devs, ctx.device)[1]
+      wg = _batch_backward(sub, devs, ctx.device, ws, [grad_outs[i] for i in wave])
+      ws = None
+      for i, g in zip(wave, wg):
+        grads[i] = g
+    return (None, None, None, None) + tuple(
         g.to(device=d, dtype=t) if need else None
-        for g, (d, t), need in zip(grads, ctx.eps_meta, ctx.needs_input_grad[3:]))
+        for g, (d, t), need in zip(grads, ctx.eps_meta, ctx.needs_input_grad[4:]))
 
 
 def batch_ckpt_every(probs):
@@ -362,7 +430,7 @@ def batch_ckpt_every(probs):
   return s + (s % 2)
 
 
-def fdtdz_batch(calls):
+def fdtdz_batch(calls, memory_budget=None):
   """Runs several fdtdz() calls, given as keyword dicts, as one batch.
 
   All simulations advance together through the batched kernels
@@ -375,6 +443,12 @@ def fdtdz_batch(calls):
   chunks and, at zz <= 64, 8 or 13 rows per tile, from a model of the work
   queue); ``launch_params`` ``tile_y`` = 8 or 13 on every zz <= 64 call fixes
   those rows, and ``graph_steps`` replays chunks of step pairs as hipGraphs.
+
+  With gradients, ``memory_budget`` (bytes; default: 90% of the free device
+  memory less room for the outputs and their cotangents) bounds the
+  checkpoint workspace held at once: a batch whose workspace exceeds it runs
+  in waves (:func:`plan_waves`), at the cost of one more forward pass for
+  every wave but the last (:class:`FdtdzBatchFunction`).
   """
   if len(calls) == 0:
     return []
@@ -389,6 +463,9 @@ def fdtdz_batch(calls):
   S = batch_ckpt_every(probs)
   for p in probs:
     p.ckpt_every = S
+  budget = memory_budget if memory_budget is not None else \
+      default_memory_budget(probs, device)
+  waves = plan_waves(probs, budget)
   eps_in = [e if isinstance(e, torch.Tensor) else torch.as_tensor(np.asarray(e))
             for e in epsilons]
-  return list(FdtdzBatchFunction.apply(probs, devs, device, *eps_in))
+  return list(FdtdzBatchFunction.apply(probs, devs, device, waves, *eps_in))

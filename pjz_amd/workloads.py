@@ -51,7 +51,10 @@ def layered_64(steps=2000, seed=0):
       launch_params=None, offset=(0, 0, 0))
 
 
-def native_512(steps=10000, seed=0, xx=512, yy=512, zz=128):
+def native_512(steps=10000, seed=0, xx=512, yy=512, zz=128, design=200):
+  """The 512x512x128 workload described above; ``xx``, ``yy``, ``zz`` and
+  the ``design`` square's side shrink it for parity runs the oracle can
+  finish (same stack, source, boundaries and snapshot schedule)."""
   X, Y, Z = xx, yy, zz
   rng = np.random.default_rng(seed)
   zs = Z / 128.0
@@ -60,9 +63,9 @@ def native_512(steps=10000, seed=0, xx=512, yy=512, zz=128):
   eps[:, :, :, :zc0] = 2.085
   yc = Y // 2
   eps[:, :, yc - 24:yc + 24, zc0:zc1] = 12.25
-  d0x, d0y = X // 2 - 100, Y // 2 - 100
-  design = np.where(rng.random((200, 200)) < 0.5, 2.085, 12.25).astype(np.float32)
-  eps[:, d0x:d0x + 200, d0y:d0y + 200, zc0:zc1] = design[None, :, :, None]
+  d0x, d0y = X // 2 - design // 2, Y // 2 - design // 2
+  pattern = np.where(rng.random((design, design)) < 0.5, 2.085, 12.25).astype(np.float32)
+  eps[:, d0x:d0x + design, d0y:d0y + design, zc0:zc1] = pattern[None, :, :, None]
   yy_, zz_ = np.meshgrid(np.arange(Y) - yc, np.arange(Z) - (zc0 + zc1) / 2, indexing="ij")
   prof = np.exp(-(yy_ ** 2) / (2 * 16.0 ** 2) - (zz_ ** 2) / (2 * 6.0 ** 2))
   prof *= 1 + 0.05 * rng.standard_normal(prof.shape)
@@ -187,19 +190,22 @@ def overlap_objective(fields, mode, omegas, output_steps, dt=0.5, monitor_x=220)
   """
   import torch
   from ._field import project
-  ef = project(fields, omegas, output_steps, dt)  # (ww, 3, xx, yy, zz) complex
-  plane = ef[:, 1:, monitor_x]                    # (ww, 2, yy, zz): Ey, Ez
+  # (ww, 2, yy, zz) complex Ey, Ez on the monitor plane (the projection is
+  # linear per cell, so only that plane is projected)
+  plane = project(fields[:, 1:, monitor_x], omegas, output_steps, dt)
   m = torch.as_tensor(mode, device=fields.device).to(plane.dtype)
   ov = torch.sum(plane * m[None], dim=(1, 2, 3))
   return torch.sum(torch.abs(ov) ** 2)
 
 
-def design_batch(steps=2000, seed=0, variants=16, xx=128, yy=128, zz=64, device=None):
+def design_batch(steps=2000, seed=0, variants=16, xx=128, yy=128, zz=64, device=None,
+                 design=None):
   """Batched inverse-design iteration: ``variants`` design variants x 2 ports
   x 2 frequencies = 4 * variants simulations of one size.
 
-  Each variant's design region (the central 5/16 of x and y, 8 z layers in
-  the core) holds eps = 2.085 + (12.25 - 2.085) * sigmoid(latent) with a
+  Each variant's design region (a centred square of side ``design``, by
+  default 5/16 of x and y, and 8 z layers in the core; 160 at 384x384x128,
+  the full-size iteration) holds eps = 2.085 + (12.25 - 2.085) * sigmoid(latent) with a
   seeded uniform(-3, 3) latent, over a SiO2 (z < zz/3) / air stack with a Si
   strip through the design.  Port 0 injects an Ey Gaussian-profile current
   plane at x = 16, port 1 at x = xx - 18; frequency k drives pjz's ramped sine
@@ -216,7 +222,7 @@ def design_batch(steps=2000, seed=0, variants=16, xx=128, yy=128, zz=64, device=
   dev = device or torch.device("cuda", 0)
   X, Y, Z = xx, yy, zz
   zc0 = Z // 2 - 4
-  dx, dy = (5 * X) // 32, (5 * Y) // 32
+  dx, dy = ((5 * X) // 32, (5 * Y) // 32) if design is None else (design // 2, design // 2)
   cx, cy = X // 2, Y // 2
   base = np.ones((3, X, Y, Z), np.float32)
   base[:, :, :, : Z // 3] = 2.085

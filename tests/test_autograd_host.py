@@ -23,6 +23,7 @@ class _Recorder:
   def __init__(self):
     self.forwards = 0
     self.backward_ws = []
+    self.batch_calls = []
 
   def forward(self, prob, dev, device):
     self.forwards += 1
@@ -35,12 +36,14 @@ class _Recorder:
 
   def batch_forward(self, probs, devs, device):
     self.forwards += 1
+    self.batch_calls.append(("forward", [p.tt for p in probs], [p.ckpt_every for p in probs]))
     outs = [torch.ones((F._lib.num_outputs(p), 3, p.ex, p.ey, p.ez)) for p in probs]
     return outs, object()
 
   def batch_backward(self, probs, devs, device, ws, grad_outs):
     self.backward_ws.append(ws)
-    return [g.sum(dim=0).to(torch.float32) * 3.0 for g in grad_outs]
+    self.batch_calls.append(("backward", [p.tt for p in probs], [p.ckpt_every for p in probs]))
+    return [g.sum(dim=0).to(torch.float32) * (3.0 + p.tt) for g, p in zip(grad_outs, probs)]
 
 
 def test_single_backward_twice_reruns_forward(monkeypatch):
@@ -60,13 +63,21 @@ def test_single_backward_twice_reruns_forward(monkeypatch):
   np.testing.assert_array_equal(g1.numpy(), (2.0 * w.sum(dim=0)).numpy())
 
 
+def _batch(steps, ckpt=4):
+  probs = (F._lib.Problem * len(steps))()
+  for i, s in enumerate(steps):
+    probs[i] = _problem(s)
+    probs[i].ckpt_every = ckpt
+  return probs
+
+
 def test_batch_backward_twice_reruns_forward(monkeypatch):
   rec = _Recorder()
   monkeypatch.setattr(F, "_batch_forward", rec.batch_forward)
   monkeypatch.setattr(F, "_batch_backward", rec.batch_backward)
-  probs = [_problem(), _problem(12)]
+  probs = _batch([10, 12])
   eps = [torch.ones((3, 4, 4, 4), dtype=torch.float64, requires_grad=True) for _ in probs]
-  outs = F.FdtdzBatchFunction.apply(probs, None, torch.device("cpu"), *eps)
+  outs = F.FdtdzBatchFunction.apply(probs, [None, None], torch.device("cpu"), [[0, 1]], *eps)
   loss = sum((o * (i + 1)).sum() for i, o in enumerate(outs))
   g1 = torch.autograd.grad(loss, eps, retain_graph=True)
   g2 = torch.autograd.grad(loss, eps)
@@ -75,3 +86,27 @@ def test_batch_backward_twice_reruns_forward(monkeypatch):
   for a, b, e in zip(g1, g2, eps):
     assert a.dtype == e.dtype and a.device == e.device
     np.testing.assert_array_equal(a.numpy(), b.numpy())
+
+
+def test_batch_waves(monkeypatch):
+  """Three waves: the forward runs every wave, only the last with
+  checkpoints; the backward takes the waves last to first and reruns the
+  forward (with checkpoints) of every wave but the last; each simulation's
+  gradient comes from its own wave."""
+  rec = _Recorder()
+  monkeypatch.setattr(F, "_batch_forward", rec.batch_forward)
+  monkeypatch.setattr(F, "_batch_backward", rec.batch_backward)
+  steps = [10, 12, 14, 16, 18]
+  probs = _batch(steps)
+  eps = [torch.ones((3, 4, 4, 4), requires_grad=True) for _ in probs]
+  waves = [[0, 1], [2], [3, 4]]
+  outs = F.FdtdzBatchFunction.apply(probs, [None] * 5, torch.device("cpu"), waves, *eps)
+  loss = sum(o.sum() for o in outs)
+  grads = torch.autograd.grad(loss, eps)
+  assert rec.batch_calls == [
+      ("forward", [10, 12], [0, 0]), ("forward", [14], [0]), ("forward", [16, 18], [4, 4]),
+      ("backward", [16, 18], [4, 4]),
+      ("forward", [14], [4]), ("backward", [14], [4]),
+      ("forward", [10, 12], [4, 4]), ("backward", [10, 12], [4, 4])]
+  for g, s in zip(grads, steps):
+    assert torch.all(g == 2 * (3.0 + s))  # two snapshots of ones, scaled by its own tt

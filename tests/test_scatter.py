@@ -142,7 +142,7 @@ def test_straight_guide_on_the_gpu(gpu):
   from pjz_amd import _mode
   xx, yy, ze = 40, 22, 20
   eps = np.ones((3, xx, yy, ze), np.float32)
-  eps[:, :, 8:14, 7:13] = 4.0
+  eps[:, :, 7:15, 8:13] = 4.0  # 8 x 5 core: no degenerate polarisations
   omega = np.array([2 * np.pi / 21, 2 * np.pi / 19])
   e = torch.as_tensor(eps, device=gpu)
   beta, exc, _, _ = _mode.mode(e[:, :1], torch.as_tensor(omega, device=gpu), 1)
