@@ -15,8 +15,9 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 Other workloads (one JSON line each): --workload grad_256 (forward +
 adjoint gradient, 256x256x128, 6000 steps), --workload batch64 (64 uneven
 simulations batched against 64 sequential calls) and --workload
-design_batch (64 simulations, forward + gradient, batched against
-sequential calls).
+design_batch (64 simulations of 128x128x64, forward + gradient, batched
+against sequential calls) and --workload design_batch_384 (the same at the
+full 384x384x128, 8000 steps, in memory-bounded waves).
 """
 
 import argparse
@@ -295,6 +296,91 @@ def bench_design(args):
   print(json.dumps(line), flush=True)
 
 
+def bench_design_full(args):
+  """The batched inverse-design iteration at its full size: 16 variants x 2
+  ports x 2 frequencies = 64 simulations of 384x384x128 cells, 160x160x8
+  design squares, 8000 steps, forward + adjoint gradient of the summed
+  overlap objective.  The gradient workspaces (~60 GB a simulation at 8000
+  steps) do not fit in HBM together, so the batch runs in waves
+  (fdtdz_batch_value_and_grad); the baseline is 64 sequential calls of the
+  default single-simulation path, each differentiated before the next."""
+  import torch
+  from pjz_amd import fdtdz, fdtdz_batch_value_and_grad, workloads
+  from pjz_amd import fdtdz_torch as F
+  steps = args.time_steps or 8000
+  calls, variant_of, monitors = workloads.design_batch(steps=steps, xx=384, yy=384, zz=128,
+                                                       design=160)
+  nv = max(variant_of) + 1
+  base_eps = [None] * nv
+  for c, v in zip(calls, variant_of):
+    base_eps[v] = c["epsilon"]
+  cells = sum(int(c["epsilon"].shape[1] * c["epsilon"].shape[2] * c["epsilon"].shape[3])
+              for c in calls)
+
+  def objective(i, out):
+    w, mon_x, mode = monitors[i]
+    return workloads.overlap_objective(out, mode, [w], calls[i]["output_steps"],
+                                       monitor_x=mon_x)
+
+  def leaves():
+    return [e.detach().clone().requires_grad_(True) for e in base_eps]
+
+  waves = []
+
+  def batched():
+    lv = leaves()
+    run = [dict(c, epsilon=lv[v]) for c, v in zip(calls, variant_of)]
+    probs = F._batch_problems(run)[0]
+    S = F.batch_ckpt_every(probs)
+    for p in probs:
+      p.ckpt_every = S
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(0.85 * (free + torch.cuda.memory_reserved() - torch.cuda.memory_allocated()))
+    waves[:] = F.plan_waves(probs, budget)
+    fdtdz_batch_value_and_grad(run, objective, memory_budget=budget)
+    return [e.grad for e in lv]
+
+  def sequential():
+    lv = leaves()
+    for i, (c, v) in enumerate(zip(calls, variant_of)):
+      out = fdtdz(**dict(c, epsilon=lv[v]))
+      objective(i, out).backward()
+      del out
+    return [e.grad for e in lv]
+
+  def timed(fn):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+      r = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / args.steps, r
+
+  for _ in range(args.warmup):
+    batched()
+  tb, gb = timed(batched)
+  print(f"design_batch_384: batched {tb:.1f} s in {len(waves)} waves", file=sys.stderr,
+        flush=True)
+  ts, gs = timed(sequential)
+  diff = max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+             for a, b in zip(gb, gs))
+  line = {"metric": METRIC, "value": round(cells * steps / tb / 1e9, 3),
+          "unit": "Gcell-updates/s (forward steps per second of forward + gradient)",
+          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+          "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+          "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+          "config": {"workload": f"design_batch_384: 16 variants x 2 ports x 2 frequencies = "
+                                 f"{len(calls)} simulations of 384x384x128 cells, 160x160x8 "
+                                 f"design squares, {steps} steps, forward + adjoint gradient "
+                                 f"of the overlap objective",
+                     "cells": cells, "time_steps": steps, "batched_s": round(tb, 3),
+                     "waves": [len(w) for w in waves],
+                     "sequential_best_s": round(ts, 3),
+                     "speedup_vs_sequential_best": round(ts / tb, 3),
+                     "grad_rel_l2_batched_vs_sequential": diff}}
+  print(json.dumps(line), flush=True)
+
+
 def bench_mode(args):
   """pjz's mode solver: microseconds per subspace iteration of the HIP kernel
   (one launch runs the loop) against the same iteration as torch ops on the
@@ -361,6 +447,8 @@ def main():
     return bench_batch(args)
   if args.workload == "design_batch":
     return bench_design(args)
+  if args.workload == "design_batch_384":
+    return bench_design_full(args)
   if args.workload == "mode":
     return bench_mode(args)
 

@@ -12,13 +12,13 @@ and ``density`` are pjz's set-up steps before it (``src/pjz/_mode.py``,
 
 from .boundaries import (absorption_mask, frequency_components, output_phases,
                          pml_sigma, ramped_sin, sampling_interval, source_amplitude)
-from .fdtdz_torch import fdtdz, fdtdz_batch
+from .fdtdz_torch import fdtdz, fdtdz_batch, fdtdz_batch_value_and_grad
 from ._density import density
 from ._epsilon import epsilon, render
 from ._field import SimParams, field
 from ._mode import mode
 from ._scatter import scatter
 
-__all__ = ["fdtdz", "fdtdz_batch", "field", "scatter", "mode", "epsilon", "render", "density",
+__all__ = ["fdtdz", "fdtdz_batch", "fdtdz_batch_value_and_grad", "field", "scatter", "mode", "epsilon", "render", "density",
            "SimParams", "absorption_mask", "pml_sigma", "ramped_sin", "sampling_interval",
            "output_phases", "frequency_components", "source_amplitude"]

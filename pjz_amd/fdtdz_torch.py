@@ -469,3 +469,38 @@ def fdtdz_batch(calls, memory_budget=None):
   eps_in = [e if isinstance(e, torch.Tensor) else torch.as_tensor(np.asarray(e))
             for e in epsilons]
   return list(FdtdzBatchFunction.apply(probs, devs, device, waves, *eps_in))
+
+
+def fdtdz_batch_value_and_grad(calls, objective, memory_budget=None):
+  """Value and epsilon-gradients of a separable objective over a batch.
+
+  ``objective(i, out)`` returns the scalar term of call ``i`` from its
+  snapshots.  The calls run in waves whose gradient workspaces fit in
+  ``memory_budget`` bytes (default: 85% of the free device memory); each
+  wave is one batched forward, its objective terms and one batched backward,
+  which accumulates into the ``.grad`` of every epsilon that requires grad.
+  Unlike one :func:`fdtdz_batch` over all calls, no wave's forward runs
+  twice: a wave's snapshots are only needed by its own terms.  Returns the
+  summed objective (a detached tensor).
+  """
+  if len(calls) == 0:
+    return torch.zeros(())
+  probs, _, device = _batch_problems(calls)
+  S = batch_ckpt_every(probs)
+  for p in probs:
+    p.ckpt_every = S
+  if memory_budget is None:
+    free, _ = torch.cuda.mem_get_info(device)
+    cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+    memory_budget = int(0.85 * (free + cached))
+  total = None
+  for wave in plan_waves(probs, memory_budget):
+    sub = [dict(calls[i], launch_params=dict(calls[i].get("launch_params") or {},
+                                             ckpt_every=S)) for i in wave]
+    outs = fdtdz_batch(sub, memory_budget=1 << 62)
+    loss = sum(objective(i, o) for i, o in zip(wave, outs))
+    if loss.requires_grad:
+      loss.backward()
+    total = loss.detach() if total is None else total + loss.detach()
+    del outs, loss
+  return total

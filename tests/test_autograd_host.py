@@ -110,3 +110,28 @@ def test_batch_waves(monkeypatch):
       ("forward", [10, 12], [4, 4]), ("backward", [10, 12], [4, 4])]
   for g, s in zip(grads, steps):
     assert torch.all(g == 2 * (3.0 + s))  # two snapshots of ones, scaled by its own tt
+
+
+def test_plan_waves_fits_the_budget():
+  """Waves cover the batch in order and each wave's workspace (from the
+  library's size query, no GPU needed) fits the budget."""
+  sizes = [(40, 30, 64), (64, 64, 128), (32, 32, 64), (80, 40, 128), (50, 50, 64)]
+  probs = (F._lib.Problem * len(sizes))()
+  for i, (X, Y, Z) in enumerate(sizes):
+    probs[i] = F.make_problem((3, X, Y, Z), 0.5, (2, 1, Y, Z), 4, (3, X, Y), (Z, 2), (4, 4),
+                              (90, 100, 5), 100, False, None, (0, 0, 0))
+    probs[i].ckpt_every = 14
+  single = [F._batch_workspace_bytes(F._subset(probs, [i])) for i in range(len(sizes))]
+  budget = int(1.5 * max(single))
+  waves = F.plan_waves(probs, budget)
+  assert [i for w in waves for i in w] == list(range(len(sizes)))
+  assert len(waves) > 1
+  for w in waves:
+    assert F._batch_workspace_bytes(F._subset(probs, w)) <= budget
+  assert F.plan_waves(probs, 10 * sum(single)) == [list(range(len(sizes)))]
+  try:
+    F.plan_waves(probs, max(single) - 1)
+  except ValueError as e:
+    assert "exceeds the memory budget" in str(e)
+  else:
+    raise AssertionError("an item larger than the budget must raise")

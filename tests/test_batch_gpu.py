@@ -158,3 +158,41 @@ def test_graph_chunks_match_direct_launches(graph_steps):
   for a, b in zip(grads0, grads1):
     # grad_eps sums edge-replicated cells with atomics: order may differ.
     assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
+
+
+def test_value_and_grad_in_waves_equals_one_batch():
+  """fdtdz_batch_value_and_grad with a budget that forces one simulation
+  per wave gives the objective and gradients of one batch over all calls;
+  so does fdtdz_batch itself under that budget (extra forward per wave)."""
+  cases = [grad_case(*c) for c in GRAD_CASES]
+  gs = [torch.as_tensor(g, device="cuda") for _, g in cases]
+
+  def run(mode):
+    eps = [torch.tensor(kw["epsilon"], device="cuda", requires_grad=True) for kw, _ in cases]
+    calls = [dict(kw, epsilon=e, launch_params=dict(ckpt_every=22)) for (kw, _), e in
+             zip(cases, eps)]
+    probs = F._batch_problems(calls)[0]
+    for p in probs:
+      p.ckpt_every = 22
+    one = max(F._batch_workspace_bytes(F._subset(probs, [i])) for i in range(len(probs)))
+    if mode == "all":
+      outs = F.fdtdz_batch(calls)
+      loss = sum((o * g).sum() for o, g in zip(outs, gs))
+      loss.backward()
+    elif mode == "waves":
+      assert len(F.plan_waves(probs, one)) == len(probs)
+      outs = F.fdtdz_batch(calls, memory_budget=one)
+      loss = sum((o * g).sum() for o, g in zip(outs, gs))
+      loss.backward()
+    else:
+      loss = F.fdtdz_batch_value_and_grad(calls, lambda i, o: (o * gs[i]).sum(),
+                                          memory_budget=one)
+    torch.cuda.synchronize()
+    return float(loss), [e.grad.cpu().numpy() for e in eps]
+
+  l0, g0 = run("all")
+  for mode in ("waves", "value_and_grad"):
+    l1, g1 = run(mode)
+    assert abs(l1 - l0) <= 1e-6 * abs(l0), mode
+    for a, b in zip(g1, g0):
+      assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b), mode
