@@ -113,6 +113,27 @@ PJZ_FDTD_API int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilo
                      void* stream);
 
 /*
+ * Forward simulation without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in pjz or fdtdz: the
+ * library-owned variant of pjz_fdtd_forward (same arguments, results and
+ * kernels).  The workspace is a library array of PJZ_FDTD_NOALLOC_MAX_BYTES
+ * per GPU, held until the process ends; a problem whose workspace
+ * (pjz_fdtd_workspace_bytes) exceeds it returns PJZ_FDTD_ESPACE, and
+ * ckpt_every must be 0.  The call allocates nothing and launches only fill,
+ * set-up and stepper kernels (an AUTOTUNE kernel id uses a cached choice
+ * for the shape, else the default kernel, instead of tuning).  Every call on
+ * a GPU uses that GPU's one array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller.
+ */
+#define PJZ_FDTD_NOALLOC_MAX_BYTES ((size_t)256 << 20)
+PJZ_FDTD_API int pjz_fdtd_forward_noalloc(const pjz_fdtd_problem* p, const float* epsilon,
+                             const float* absorption_mask, const float* pml_kappa,
+                             const float* pml_sigma, const float* pml_alpha,
+                             const float* source_field, const float* source_waveform,
+                             float* out, void* stream);
+
+/*
  * Batched forward: n independent simulations of uneven size advanced together
  * (persistent two-step launches over a prefix-summed (simulation, tile) work
  * queue), each with the inputs and outputs of pjz_fdtd_forward.  Pointer

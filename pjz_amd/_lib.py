@@ -22,6 +22,7 @@ EXPORTS = (
     "pjz_fdtd_num_outputs",
     "pjz_fdtd_workspace_bytes",
     "pjz_fdtd_forward",
+    "pjz_fdtd_forward_noalloc",
     "pjz_fdtd_backward",
     "pjz_fdtd_batch_workspace_bytes",
     "pjz_fdtd_batch_forward",
@@ -80,6 +81,8 @@ def load():
     lib.pjz_fdtd_workspace_bytes.argtypes = [P, ctypes.POINTER(sz)]
     lib.pjz_fdtd_forward.restype = ctypes.c_int
     lib.pjz_fdtd_forward.argtypes = [P] + [vp] * 8 + [vp, sz, vp]
+    lib.pjz_fdtd_forward_noalloc.restype = ctypes.c_int
+    lib.pjz_fdtd_forward_noalloc.argtypes = [P] + [vp] * 8 + [vp]
     lib.pjz_fdtd_backward.restype = ctypes.c_int
     lib.pjz_fdtd_backward.argtypes = [P] + [vp] * 9 + [vp, sz, vp]
     lib.pjz_fdtd_batch_workspace_bytes.restype = ctypes.c_int

@@ -131,6 +131,35 @@ hipError_t run_steps(const pjz::Geom& g, const pjz_fdtd_problem& p, const float*
                      const pjz::Layout& L, int b, int end, int* cur, float* capture,
                      hipStream_t s);
 
+// ------------------------------------------------- library-held scratch
+// Workspace of pjz_fdtd_forward_noalloc: one array per GPU, held until the
+// process ends.  In host code `scratch` only names the symbol; its device
+// address on the current GPU comes from hipGetSymbolAddress, looked up once
+// per device and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = PJZ_FDTD_NOALLOC_MAX_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current GPU's scratch array (nullptr and a HIP error
+// on failure).
+hipError_t scratch_address(void** ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 hipError_t clear_state(const pjz::Geom& g, const pjz::Layout& L, char* ws, hipStream_t s) {
   hipError_t e = pjz::launch_zero((float*)(ws + L.E0), (L.E1 - L.E0) / sizeof(float), s);
   if (e == hipSuccess && g.NP > 0)
@@ -406,6 +435,35 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
   }
   if (e != hipSuccess) return hip_fail(e, "step launch");
   return PJZ_FDTD_OK;
+}
+
+int pjz_fdtd_forward_noalloc(const pjz_fdtd_problem* p, const float* epsilon,
+                             const float* absorption_mask, const float* pml_kappa,
+                             const float* pml_sigma, const float* pml_alpha,
+                             const float* source_field, const float* source_waveform,
+                             float* out, void* stream) {
+  int rc = check(p);
+  if (rc) return rc;
+  if (p->ckpt_every > 0)
+    return fail(PJZ_FDTD_EINVAL,
+                "forward_noalloc runs without checkpoints: ckpt_every must be 0, got %d",
+                p->ckpt_every);
+  const size_t need = pjz::make_layout(geom_of(*p), *p).total;
+  if (need > kScratchBytes)
+    return fail(PJZ_FDTD_ESPACE,
+                "forward_noalloc: a (%d,%d,%d) domain needs %zu bytes of workspace, above the "
+                "library's %zu-byte scratch array (use pjz_fdtd_forward with a workspace)",
+                p->xx, p->yy, p->zz, need, kScratchBytes);
+  void* ws = nullptr;
+  hipError_t e = scratch_address(&ws);
+  if (e != hipSuccess) return hip_fail(e, "forward_noalloc: scratch address");
+  // Autotuning would synchronise and time candidates; use a cached choice
+  // for the shape if there is one, else the default kernel.
+  pjz_fdtd_problem q = *p;
+  Tuned t;
+  if (q.kernel == PJZ_FDTD_KERNEL_AUTOTUNE && !tuned_choice(q, &t)) q.kernel = PJZ_FDTD_KERNEL_AUTO;
+  return pjz_fdtd_forward(&q, epsilon, absorption_mask, pml_kappa, pml_sigma, pml_alpha,
+                          source_field, source_waveform, out, ws, kScratchBytes, stream);
 }
 
 }  // extern "C"
