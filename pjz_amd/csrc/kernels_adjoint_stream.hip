@@ -65,7 +65,9 @@ struct AdjPlane {
   float lp[2][CPL];   // psi_E cotangent at x (PML cells)
   float d[3][CPL];    // D at x (owner cells)
   float g[3][CPL];    // G at x (owner cells)
-  float kup[2][CPL];  // lK x, z at (x, y+1) (top row of the workgroup only)
+  float kue[2][CPL];  // lE x, z at (x, y+1) (top row of the workgroup only)
+  float kuc[2][CPL];  // cb x, z at (x, y+1): lK = kuc kue, formed where used so
+                      // the fetch does not wait for its own loads
   float sig[3];       // absorber at (x, y) (owner cells)
 };
 
@@ -86,16 +88,10 @@ __device__ inline void adj_fetch(AdjPlane<CPL>& P, const AdjStreamArgs& a, int x
   }
   if (w == WY) {
     const bool v = active && y + 1 < Y;
-    float e0[CPL], e2[CPL], c0[CPL], c2[CPL];
-    load_col<CPL, vec>(e0, a.lEin + (o + Z), zb, Z, v);
-    load_col<CPL, vec>(e2, a.lEin + (2 * N + o + Z), zb, Z, v);
-    load_col<CPL, vec>(c0, a.cb + (o + Z), zb, Z, v);
-    load_col<CPL, vec>(c2, a.cb + (2 * N + o + Z), zb, Z, v);
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      P.kup[0][c] = c0[c] * e0[c];
-      P.kup[1][c] = c2[c] * e2[c];
-    }
+    load_col<CPL, vec>(P.kue[0], a.lEin + (o + Z), zb, Z, v);
+    load_col<CPL, vec>(P.kue[1], a.lEin + (2 * N + o + Z), zb, Z, v);
+    load_col<CPL, vec>(P.kuc[0], a.cb + (o + Z), zb, Z, v);
+    load_col<CPL, vec>(P.kuc[1], a.cb + (2 * N + o + Z), zb, Z, v);
   }
   const int pb = (x * Y + y) * a.NP;
 #pragma unroll
@@ -201,8 +197,8 @@ __device__ __forceinline__ void adj_tile(const AdjStreamArgs& a, int tile, AdjLd
     } else {
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        kyp[0][c] = cur.kup[0][c];
-        kyp[1][c] = cur.kup[1][c];
+        kyp[0][c] = cur.kuc[0][c] * cur.kue[0][c];
+        kyp[1][c] = cur.kuc[1][c] * cur.kue[1][c];
       }
     }
 

@@ -127,7 +127,9 @@ struct Adj2Plane {
   float lh[3][CPL];   // lH at x
   float lu[2][CPL];   // psi_H cotangent at x
   float lp[2][CPL];   // psi_E cotangent at x
-  float kup[2][CPL];  // lK x, z at (x, y+1) (top row only)
+  float kue[2][CPL];  // lE x, z at (x, y+1) (top row only)
+  float kuc[2][CPL];  // cb x, z at (x, y+1): lK = kuc kue, formed where used so
+                      // the fetch does not wait for its own loads
   float sig[3];       // absorber at (x, y)
 };
 
@@ -160,16 +162,10 @@ __device__ inline void adj2_fetch(Adj2Plane<CPL>& P, const Adj2Args& a, int x, i
   }
   if (top) {
     const bool v = active && yc + 1 < Y;
-    float e0[CPL], e2[CPL], c0[CPL], c2[CPL];
-    load_col<CPL, true>(e0, a.lEin + (o + Z), zb, Z, v);
-    load_col<CPL, true>(e2, a.lEin + (2 * N + o + Z), zb, Z, v);
-    load_col<CPL, true>(c0, a.cb + (o + Z), zb, Z, v);
-    load_col<CPL, true>(c2, a.cb + (2 * N + o + Z), zb, Z, v);
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      P.kup[0][c] = c0[c] * e0[c];
-      P.kup[1][c] = c2[c] * e2[c];
-    }
+    load_col<CPL, true>(P.kue[0], a.lEin + (o + Z), zb, Z, v);
+    load_col<CPL, true>(P.kue[1], a.lEin + (2 * N + o + Z), zb, Z, v);
+    load_col<CPL, true>(P.kuc[0], a.cb + (o + Z), zb, Z, v);
+    load_col<CPL, true>(P.kuc[1], a.cb + (2 * N + o + Z), zb, Z, v);
   }
   const int pb = (x * Y + yc) * a.NP;
 #pragma unroll
@@ -210,7 +206,6 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
   const bool act = row_valid && l1;
   const int N = a.N, Y = a.Y, Z = a.Z;
   const float dt = a.dt;
-  if (x + 1 <= xb) adj2_fetch<CPL>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, zc);
   // This plane's D^n (level 1 at x) and G, D^(n-1) (level 2 at x-1), owned
   // rows only: issued now, used in phases B and C.
   float d1[3][CPL], g[3][CPL], d2[3][CPL];
@@ -224,6 +219,9 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
       load_col<CPL, true>(d2[k], a.D2 + (k * N + om), zb, Z, st);
     }
   }
+  // The next plane's inputs are requested after these: waiting for D and G
+  // (in-order completion) then leaves the prefetch in flight.
+  if (x + 1 <= xb) adj2_fetch<CPL>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, zc);
 
   // ---- A: level-1 lH at plane x
   float K1[3][CPL];
@@ -250,8 +248,8 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
   } else {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      Ky[0][c] = cur.kup[0][c];
-      Ky[1][c] = cur.kup[1][c];
+      Ky[0][c] = cur.kuc[0][c] * cur.kue[0][c];
+      Ky[1][c] = cur.kuc[1][c] * cur.kue[1][c];
     }
   }
   float h1[3][CPL], lu1[2][CPL];

@@ -282,8 +282,8 @@ __device__ __forceinline__ void store_rare(const Stream2Args& a, bool snap, bool
     if (slot >= 0 && x >= r->wx0 && x < r->wx0 + r->wex && y >= r->wy0 && y < r->wy0 + r->wey) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        float* op = r->out + ((((long long)slot * 3 + k) * r->wex + (x - r->wx0)) * r->wey +
-                              (y - r->wy0)) * (long long)r->wez;
+        gfloat* op = to_global(r->out) + ((((long long)slot * 3 + k) * r->wex + (x - r->wx0)) *
+                                              r->wey + (y - r->wy0)) * (long long)r->wez;
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const int z = zb + c;
@@ -293,14 +293,14 @@ __device__ __forceinline__ void store_rare(const Stream2Args& a, bool snap, bool
     }
   }
   if (cap && r->cap) {
-    float* base = r->cap + (long long)(step - r->cap_begin) * 3 * a.N;
+    gfloat* base = to_global(r->cap) + (long long)(step - r->cap_begin) * 3 * a.N;
     const int o = (x * a.Y + y) * a.Z;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       float d[CPL];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) d[c] = eo[k][c] - en[k][c];
-      store_col<CPL, vec>(base + (k * a.N + o), d, zb, a.Z);
+      store_col_global<CPL, vec>(base + (k * a.N + o), d, zb, a.Z);
     }
   }
 }

@@ -61,6 +61,38 @@ __device__ inline void store_col(float* __restrict__ p, const float (&v)[CPL], i
   }
 }
 
+// Global (address space 1) views of generic device pointers.  Stores through
+// them compile to global_store, not flat_store: a pending flat operation makes
+// the compiler wait for every outstanding vector memory operation (vmcnt(0))
+// at the next join, which at a loop's end also waits for the stores just
+// issued, not only for the prefetched loads the next iteration needs.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) f32x2 gf32x2;
+typedef __attribute__((address_space(1))) f32x4 gf32x4;
+__device__ inline gfloat* to_global(float* p) { return (gfloat*)p; }
+
+// store_col through a global pointer.
+template <int CPL, bool vec>
+__device__ inline void store_col_global(gfloat* __restrict__ p, const float (&v)[CPL], int zb,
+                                        int Z) {
+  if (zb >= Z) return;
+  if (vec) {
+    if constexpr (CPL == 4) {
+      f32x4 t = {v[0], v[1], v[2], v[3]};
+      __builtin_nontemporal_store(t, (gf32x4*)(p + zb));
+    } else if constexpr (CPL == 2) {
+      f32x2 t = {v[0], v[1]};
+      __builtin_nontemporal_store(t, (gf32x2*)(p + zb));
+    } else {
+      __builtin_nontemporal_store(v[0], p + zb);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+      if (zb + c < Z) p[zb + c] = v[c];
+  }
+}
+
 // Raw buffer access: a 128-bit resource (base, size in bytes) in scalar
 // registers plus 32-bit byte offsets, one vector (VGPR) and one uniform
 // (SGPR).  An offset at or past the resource's size reads 0 and drops the
