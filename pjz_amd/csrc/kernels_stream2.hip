@@ -139,14 +139,14 @@ __device__ inline void add_source2(const Stream2Args& a, int step, int x, int y,
   }
 }
 
-template <int CPL>
+template <int CPL, int W = 64>
 __device__ inline void h_row(float (&hn)[3][CPL], float (&psHn)[2][CPL], const float (&e0)[3][CPL],
                              const float (&e1)[3][CPL], const float (&eup)[2][CPL],
                              const float (&hold)[3][CPL], const float (&psH)[2][CPL],
                              const ZCoef<CPL>& zc, int zb, int Z, float dt, bool valid) {
   float exz[CPL], eyz[CPL];
-  z_next<CPL>(exz, e0[0], zb, Z);
-  z_next<CPL>(eyz, e0[1], zb, Z);
+  z_next<CPL, W>(exz, e0[0], zb, Z);
+  z_next<CPL, W>(eyz, e0[1], zb, Z);
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const float dzEy = eyz[c] - e0[1][c];
@@ -162,7 +162,7 @@ __device__ inline void h_row(float (&hn)[3][CPL], float (&psHn)[2][CPL], const f
   }
 }
 
-template <int CPL>
+template <int CPL, int W = 64>
 __device__ inline void e_row(float (&en)[3][CPL], float (&psEn)[2][CPL], const float (&e)[3][CPL],
                              const float (&hn)[3][CPL], const float (&hdn)[2][CPL],
                              const float (&hp)[2][CPL], const float (&cb)[3][CPL],
@@ -170,8 +170,8 @@ __device__ inline void e_row(float (&en)[3][CPL], float (&psEn)[2][CPL], const f
                              const ZCoef<CPL>& zc, const Stream2Args& a, int step, int x, int y,
                              int zb, int lane, bool valid) {
   float hxz[CPL], hyz[CPL];
-  z_prev<CPL>(hxz, hn[0], lane);
-  z_prev<CPL>(hyz, hn[1], lane);
+  z_prev<CPL, W>(hxz, hn[0], lane);
+  z_prev<CPL, W>(hyz, hn[1], lane);
   float cx[CPL], cy[CPL], cz[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
@@ -220,7 +220,7 @@ struct Carry2 {
   float sigm[3];
 };
 
-template <int CPL, bool vec>
+template <int CPL, bool vec, int RPW = 1>
 __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int yc, bool top,
                               int zb, bool active, bool e_rows, const ZCoef<CPL>& zc) {
   const int N = a.N, Z = a.Z;
@@ -265,9 +265,16 @@ __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y
       P.ps[3][c] = ph && e_rows ? a.psiIn[3 * a.P + pi] : 0.0f;
     }
   }
+  // One row per wave: the absorber value is wave-uniform (scalar load).
+  // Two rows per wave: one value per half.
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
-    P.sig[k] = active && e_rows ? uniform_load(a.mask, (k * a.X + x) * a.Y + yc) : 0.0f;
+  for (int k = 0; k < 3; ++k) {
+    const int mi = (k * a.X + x) * a.Y + yc;
+    if constexpr (RPW == 1)
+      P.sig[k] = active && e_rows ? uniform_load(a.mask, mi) : 0.0f;
+    else
+      P.sig[k] = active && e_rows ? a.mask[mi] : 0.0f;
+  }
 }
 
 // Writes the snapshot of E after `step` (en) and/or the capture of its change
@@ -305,14 +312,16 @@ __device__ __forceinline__ void store_rare(const Stream2Args& a, bool snap, bool
   }
 }
 
-// Row exchange buffers; a row's cell c of lane l sits at [c * 64 + l], so the
-// 64 lanes of an access hit consecutive banks.
-template <int CPL, int WY>
+// Row exchange buffers; a row's cell c of lane l (of the LPR = 64 / RPW
+// lanes that hold the row) sits at [c * LPR + l], so the lanes of an access
+// hit consecutive banks.
+template <int CPL, int WY, int RPW = 1>
 struct Lds2 {
-  float sE1[2][WY + 3][2][64 * CPL];
-  float sH1[WY + 3][2][64 * CPL];
-  float sE2[2][WY + 3][2][64 * CPL];
-  float sH2[WY + 3][2][64 * CPL];
+  static constexpr int LPR = 64 / RPW;
+  float sE1[2][WY + 3][2][LPR * CPL];
+  float sH1[WY + 3][2][LPR * CPL];
+  float sE2[2][WY + 3][2][LPR * CPL];
+  float sH2[WY + 3][2][LPR * CPL];
 };
 
 // Level-2 outputs (E, H at Eout + hoff, new psi) written through buffer
@@ -338,14 +347,18 @@ __device__ inline Out2 make_out2(const Stream2Args& a) {
 // sE2 slot written this plane (the other slot holds plane x-1) and the sE1
 // slot read this plane (input Ex, Ez of plane x, published one plane early
 // from the prefetched registers, so level-1 H needs no barrier of its own).
-template <int CPL, int WY, bool vec>
-__device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2<CPL, WY>& L,
-                                       Carry2<CPL>& s,
+// `w` is the row within the tile (rows y0-2 .. y0+WY) and `lane` the lane
+// within the row's LPR lanes; with two rows per wave (RPW = 2) both vary
+// within a wave.
+template <int CPL, int WY, bool vec, int RPW = 1>
+__device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O,
+                                       Lds2<CPL, WY, RPW>& L, Carry2<CPL>& s,
                                        const Plane2<CPL>& cur, Plane2<CPL>& nxt,
                                        const ZCoef<CPL>& zc, int x, int xa, int xb, int w,
                                        int y, int yc, int lane, int zb, bool row_valid,
                                        bool lane_ok, int PAR) {
   constexpr int NW = WY + 3;
+  constexpr int LPR = 64 / RPW;
   const bool e_rows = w >= 1;
   const bool h2_rows = w >= 1 && w <= WY + 1;
   const bool own = w >= 2 && w <= WY + 1;
@@ -354,22 +367,23 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
   const bool act = row_valid && l1;
   const int Z = a.Z, N = a.N;
   if (x + 1 <= xb)
-    fetch2<CPL, vec>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, e_rows, zc);
+    fetch2<CPL, vec, RPW>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, e_rows,
+                          zc);
 
   // ---- A: level-1 H at plane x (input E of the row above from sE1[PAR];
   // this row's input E of plane x+1 goes to sE1[PAR ^ 1], read next plane
   // after this plane's barriers)
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sE1[PAR ^ 1][w][0][c * 64 + lane] = cur.e[0][c];
-    L.sE1[PAR ^ 1][w][1][c * 64 + lane] = cur.e[2][c];
+    L.sE1[PAR ^ 1][w][0][c * LPR + lane] = cur.e[0][c];
+    L.sE1[PAR ^ 1][w][1][c * LPR + lane] = cur.e[2][c];
   }
   float eup[2][CPL];
   if (w < NW - 1) {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      eup[0][c] = L.sE1[PAR][w + 1][0][c * 64 + lane];
-      eup[1][c] = L.sE1[PAR][w + 1][1][c * 64 + lane];
+      eup[0][c] = L.sE1[PAR][w + 1][0][c * LPR + lane];
+      eup[1][c] = L.sE1[PAR][w + 1][1][c * LPR + lane];
     }
   } else {
 #pragma unroll
@@ -386,12 +400,12 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
       psH[0][c] = cur.ps[0][c];
       psH[1][c] = cur.ps[1][c];
     }
-    h_row<CPL>(hn1, psH1, s.e0, cur.e, eup, cur.h, psH, zc, zb, Z, a.dt, act);
+    h_row<CPL, LPR>(hn1, psH1, s.e0, cur.e, eup, cur.h, psH, zc, zb, Z, a.dt, act);
   }
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sH1[w][0][c * 64 + lane] = hn1[0][c];
-    L.sH1[w][1][c * 64 + lane] = hn1[2][c];
+    L.sH1[w][0][c * LPR + lane] = hn1[0][c];
+    L.sH1[w][1][c * LPR + lane] = hn1[2][c];
   }
   lds_barrier();
 
@@ -401,14 +415,14 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
     float hdn[2][CPL], hp[2][CPL], psE[2][CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      hdn[0][c] = L.sH1[w - 1][0][c * 64 + lane];
-      hdn[1][c] = L.sH1[w - 1][1][c * 64 + lane];
+      hdn[0][c] = L.sH1[w - 1][0][c * LPR + lane];
+      hdn[1][c] = L.sH1[w - 1][1][c * LPR + lane];
       hp[0][c] = s.h1m[1][c];
       hp[1][c] = s.h1m[2][c];
       psE[0][c] = cur.ps[2][c];
       psE[1][c] = cur.ps[3][c];
     }
-    e_row<CPL>(en1, psE1, s.e0, hn1, hdn, hp, cur.cb, cur.sig, psE, zc, a, a.n, x, y, zb, lane,
+    e_row<CPL, LPR>(en1, psE1, s.e0, hn1, hdn, hp, cur.cb, cur.sig, psE, zc, a, a.n, x, y, zb, lane,
                act);
     if ((a.flags & (SNAP1 | CAPTURE)) && own && act && x >= xa && x < xb)
       store_rare<CPL, vec>(a, a.flags & SNAP1, a.flags & CAPTURE, a.n, x, y, zb, en1, s.e0);
@@ -421,20 +435,20 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
   }
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sE2[PAR][w][0][c * 64 + lane] = en1[0][c];
-    L.sE2[PAR][w][1][c * 64 + lane] = en1[2][c];
+    L.sE2[PAR][w][0][c * LPR + lane] = en1[0][c];
+    L.sE2[PAR][w][1][c * LPR + lane] = en1[2][c];
   }
   float hn2[3][CPL], psH2[2][CPL];
   if (h2_rows && l2) {
     float e2up[2][CPL], psHm[2][CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      e2up[0][c] = L.sE2[PAR ^ 1][w + 1][0][c * 64 + lane];
-      e2up[1][c] = L.sE2[PAR ^ 1][w + 1][1][c * 64 + lane];
+      e2up[0][c] = L.sE2[PAR ^ 1][w + 1][0][c * LPR + lane];
+      e2up[1][c] = L.sE2[PAR ^ 1][w + 1][1][c * LPR + lane];
       psHm[0][c] = s.psm[0][c];
       psHm[1][c] = s.psm[1][c];
     }
-    h_row<CPL>(hn2, psH2, s.e1m, en1, e2up, s.h1m, psHm, zc, zb, Z, a.dt, row_valid);
+    h_row<CPL, LPR>(hn2, psH2, s.e1m, en1, e2up, s.h1m, psHm, zc, zb, Z, a.dt, row_valid);
   } else {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
@@ -444,8 +458,8 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
   }
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sH2[w][0][c * 64 + lane] = hn2[0][c];
-    L.sH2[w][1][c * 64 + lane] = hn2[2][c];
+    L.sH2[w][0][c * LPR + lane] = hn2[0][c];
+    L.sH2[w][1][c * LPR + lane] = hn2[2][c];
   }
   lds_barrier();
 
@@ -456,12 +470,12 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
     float hdn[2][CPL], psEm[2][CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      hdn[0][c] = L.sH2[w - 1][0][c * 64 + lane];
-      hdn[1][c] = L.sH2[w - 1][1][c * 64 + lane];
+      hdn[0][c] = L.sH2[w - 1][0][c * LPR + lane];
+      hdn[1][c] = L.sH2[w - 1][1][c * LPR + lane];
       psEm[0][c] = s.psm[2][c];
       psEm[1][c] = s.psm[3][c];
     }
-    e_row<CPL>(en2, psE2, s.e1m, hn2, hdn, s.h2p, s.cbm, s.sigm, psEm, zc, a, a.n + 1, x - 1, y,
+    e_row<CPL, LPR>(en2, psE2, s.e1m, hn2, hdn, s.h2p, s.cbm, s.sigm, psEm, zc, a, a.n + 1, x - 1, y,
                zb, lane, true);
     if (st && (a.flags & (SNAP2 | CAPTURE)))
       store_rare<CPL, vec>(a, a.flags & SNAP2, a.flags & CAPTURE, a.n + 1, x - 1, y, zb, en2,
@@ -521,12 +535,17 @@ __device__ __forceinline__ void plane2(const Stream2Args& a, const Out2& O, Lds2
 }
 
 // One tile (WY rows x LX planes) of one simulation for steps a.n, a.n+1.
-template <int CPL, int WY, bool vec>
-__device__ __forceinline__ void stream2_tile(const Stream2Args& a, int tile, Lds2<CPL, WY>& L) {
+template <int CPL, int WY, bool vec, int RPW = 1>
+__device__ __forceinline__ void stream2_tile(const Stream2Args& a, int tile,
+                                             Lds2<CPL, WY, RPW>& L) {
   constexpr int NW = WY + 3;
+  constexpr int LPR = 64 / RPW;
   const int Z = a.Z;
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // Row w of the tile and lane within the row: one row per wave (the row is
+  // wave-uniform), or two rows per wave, one per 32-lane half.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = RPW == 1 ? (int)(threadIdx.x & 63) : (int)(threadIdx.x & (LPR - 1));
+  const int w = RPW == 1 ? wave : wave * RPW + (int)((threadIdx.x & 63) / LPR);
   const int zb = lane * CPL;
   const int ty = tile % a.nty, tx = tile / a.nty;
   const int y = ty * WY - 2 + w;
@@ -566,33 +585,33 @@ __device__ __forceinline__ void stream2_tile(const Stream2Args& a, int tile, Lds
     for (int k = 0; k < 3; ++k) load_col<CPL, vec>(s.e0[k], a.Ein + (k * a.N + o), zb, Z, row_valid);
   }
   Plane2<CPL> cur, nxt;
-  fetch2<CPL, vec>(cur, a, xs, yc, w == NW - 1, zb, row_valid, w >= 1, zc);
+  fetch2<CPL, vec, RPW>(cur, a, xs, yc, w == NW - 1, zb, row_valid, w >= 1, zc);
   // Input Ex, Ez of the first plane for the row below's level-1 H.
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
-    L.sE1[0][w][0][c * 64 + lane] = s.e0[0][c];
-    L.sE1[0][w][1][c * 64 + lane] = s.e0[2][c];
+    L.sE1[0][w][0][c * LPR + lane] = s.e0[0][c];
+    L.sE1[0][w][1][c * LPR + lane] = s.e0[2][c];
   }
   lds_barrier();
   const Out2 O = make_out2(a);
   const bool lane_ok = zb < Z;
   int par = 0;
   for (int x = xs; x <= xb; ++x) {
-    plane2<CPL, WY, vec>(a, O, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid,
+    plane2<CPL, WY, vec, RPW>(a, O, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid,
                          lane_ok, par);
     cur = nxt;
     par ^= 1;
   }
 }
 
-template <int CPL, int WY, bool vec>
-__global__ __launch_bounds__(64 * (WY + 3)) void k_stream2(Stream2Args a) {
-  __shared__ Lds2<CPL, WY> L;
+template <int CPL, int WY, bool vec, int RPW = 1>
+__global__ __launch_bounds__(64 * (WY + 3) / RPW) void k_stream2(Stream2Args a) {
+  __shared__ Lds2<CPL, WY, RPW> L;
   // Consecutive workgroup ids go round-robin over the 8 XCDs; remap so that
   // tiles adjacent in y (which share ghost rows) run on the same XCD.
   int tile = blockIdx.x;
   if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
-  stream2_tile<CPL, WY, vec>(a, tile, L);
+  stream2_tile<CPL, WY, vec, RPW>(a, tile, L);
 }
 
 __device__ inline int uniform_load_int(const int* p) {
@@ -635,9 +654,11 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_batch(
   }
 }
 
-template <int CPL, int WY, bool vec>
+template <int CPL, int WY, bool vec, int RPW = 1>
 static void launch2(Stream2Args& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_stream2<CPL, WY, vec>), dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
+  static_assert((WY + 3) % RPW == 0, "rows of a tile must fill whole waves");
+  hipLaunchKernelGGL((k_stream2<CPL, WY, vec, RPW>), dim3(a.ntiles), dim3(64 * (WY + 3) / RPW),
+                     0, s, a);
 }
 
 // Whether the two-step kernel handles this problem (vector loads needed).
@@ -651,7 +672,19 @@ static bool fits_rsrc(const Geom& g) {
 
 bool stream2_supported(const Geom& g) {
   return fits_rsrc(g) && g.Z <= 256 &&
-         ((g.Z <= 64) || (g.Z <= 128 && g.Z % 2 == 0) || (g.Z % 4 == 0));
+         ((g.Z <= 64) || (g.Z <= 96 && g.Z % 3 == 0) || (g.Z <= 128 && g.Z % 2 == 0) ||
+          (g.Z % 4 == 0));
+}
+
+// Heights 64 < zz <= 96 that are multiples of 3 (pjz's default budget is
+// 128 - 32 = 96) run three cells per lane with two rows per wave (32 lanes
+// each): every lane holds cells, where two cells per lane would leave a
+// quarter of the lanes empty at 96.  tile_y 4, 8, 12 or 13 asks for the
+// two-cells-per-lane kernel instead (even heights).
+static bool use_cpl3(const Geom& g, const pjz_fdtd_problem& p) {
+  if (g.Z <= 64 || g.Z > 96 || g.Z % 3 != 0) return false;
+  const bool cpl2_asked = p.tile_y == 4 || p.tile_y == 8 || p.tile_y == 12 || p.tile_y == 13;
+  return !(cpl2_asked && g.Z % 2 == 0);
 }
 
 // Runs steps [r.begin, r.end) two at a time; an odd last step uses the
@@ -730,10 +763,10 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
                        const Src& src, const OutWin& win, char* ws, const Layout& L,
                        const StepRange& r, hipStream_t s) {
   if (!stream2_supported(g)) return hipErrorInvalidValue;
-  const int cpl = g.Z <= 64 ? 1 : (g.Z <= 128 ? 2 : 4);
+  const int cpl = g.Z <= 64 ? 1 : use_cpl3(g, p) ? 3 : (g.Z <= 128 ? 2 : 4);
   // Row tiles with a compiled kernel: 8 (4 at CPL = 4); CPL = 2 also 4, 12,
   // 13; CPL = 1 also 13 (16 waves, which its register use lets share a CU).
-  int WY = cpl == 4 ? 4 : 8;
+  int WY = cpl == 4 ? 4 : cpl == 3 ? 13 : 8;
   if (cpl == 2 && (p.tile_y == 4 || p.tile_y == 12 || p.tile_y == 13)) WY = p.tile_y;
   // One cell per lane: 13-row tiles (16 waves) unless they cannot give every
   // CU one even at 8-plane chunks (measured at 512^2, 300^2 and 200^2 x 64:
@@ -772,7 +805,8 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
     a.n = n;
     a.flags = (slot(n) >= 0 ? SNAP1 : 0) | (slot(n + 1) >= 0 ? SNAP2 : 0) |
               (r.capture ? CAPTURE : 0);
-    if (cpl == 1 && WY == 13) launch2<1, 13, true>(a, s);
+    if (cpl == 3) launch2<3, 13, true, 2>(a, s);
+    else if (cpl == 1 && WY == 13) launch2<1, 13, true>(a, s);
     else if (cpl == 1) launch2<1, 8, true>(a, s);
     else if (cpl == 4) launch2<4, 4, true>(a, s);
     else if (WY == 4) launch2<2, 4, true>(a, s);

@@ -28,6 +28,10 @@ __device__ inline void load_col(float (&v)[CPL], const float* __restrict__ p, in
       const f32x2* q = reinterpret_cast<const f32x2*>(p + zb);
       const f32x2 t = NT ? __builtin_nontemporal_load(q) : *q;
       v[0] = t.x; v[1] = t.y;
+    } else if constexpr (CPL == 3) {
+      // three 4-byte-aligned floats: merged into one 96-bit load
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = NT ? __builtin_nontemporal_load(p + zb + c) : p[zb + c];
     } else {
       v[0] = NT ? __builtin_nontemporal_load(p + zb) : p[zb];
     }
@@ -51,6 +55,9 @@ __device__ inline void store_col(float* __restrict__ p, const float (&v)[CPL], i
     } else if constexpr (CPL == 2) {
       f32x2 t = {v[0], v[1]};
       __builtin_nontemporal_store(t, reinterpret_cast<f32x2*>(p + zb));
+    } else if constexpr (CPL == 3) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) __builtin_nontemporal_store(v[c], p + zb + c);
     } else {
       __builtin_nontemporal_store(v[0], p + zb);
     }
@@ -83,6 +90,9 @@ __device__ inline void store_col_global(gfloat* __restrict__ p, const float (&v)
     } else if constexpr (CPL == 2) {
       f32x2 t = {v[0], v[1]};
       __builtin_nontemporal_store(t, (gf32x2*)(p + zb));
+    } else if constexpr (CPL == 3) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) __builtin_nontemporal_store(v[c], p + zb + c);
     } else {
       __builtin_nontemporal_store(v[0], p + zb);
     }
@@ -112,6 +122,7 @@ __device__ inline rsrc_t make_rsrc(const void* base, unsigned bytes) {
 }
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline float bload1(rsrc_t r, unsigned voff, unsigned soff) {
@@ -131,6 +142,11 @@ __device__ inline void bload(float (&v)[CPL], rsrc_t r, unsigned voff, unsigned 
     v[1] = __builtin_bit_cast(float, t.y);
     v[2] = __builtin_bit_cast(float, t.z);
     v[3] = __builtin_bit_cast(float, t.w);
+  } else if constexpr (CPL == 3) {
+    const u32x3 t = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    v[0] = __builtin_bit_cast(float, t.x);
+    v[1] = __builtin_bit_cast(float, t.y);
+    v[2] = __builtin_bit_cast(float, t.z);
   } else if constexpr (CPL == 2) {
     const u32x2 t = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
     v[0] = __builtin_bit_cast(float, t.x);
@@ -149,6 +165,12 @@ __device__ inline void bstore(const float (&v)[CPL], rsrc_t r, unsigned voff, un
     t.z = __builtin_bit_cast(unsigned, v[2]);
     t.w = __builtin_bit_cast(unsigned, v[3]);
     __builtin_amdgcn_raw_buffer_store_b128(t, r, voff, soff, kStoreNT);
+  } else if constexpr (CPL == 3) {
+    u32x3 t;
+    t.x = __builtin_bit_cast(unsigned, v[0]);
+    t.y = __builtin_bit_cast(unsigned, v[1]);
+    t.z = __builtin_bit_cast(unsigned, v[2]);
+    __builtin_amdgcn_raw_buffer_store_b96(t, r, voff, soff, kStoreNT);
   } else if constexpr (CPL == 2) {
     u32x2 t;
     t.x = __builtin_bit_cast(unsigned, v[0]);
@@ -165,10 +187,11 @@ __device__ inline void bstore(const float (&v)[CPL], rsrc_t r, unsigned voff, un
 typedef const __attribute__((address_space(4))) float* const_fptr;
 __device__ inline float uniform_load(const float* p, int i) { return ((const_fptr)p)[i]; }
 
-// Cell z+1 for each of the lane's cells (0 at and beyond Z).
-template <int CPL>
+// Cell z+1 for each of the lane's cells (0 at and beyond Z).  A z column
+// spans W lanes (64, or 32 when a wave holds two rows).
+template <int CPL, int W = 64>
 __device__ inline void z_next(float (&o)[CPL], const float (&v)[CPL], int zb, int Z) {
-  float up = __shfl_down(v[0], 1, 64);
+  float up = __shfl_down(v[0], 1, W);
 #pragma unroll
   for (int c = 0; c < CPL - 1; ++c) o[c] = v[c + 1];
   o[CPL - 1] = up;
@@ -177,10 +200,11 @@ __device__ inline void z_next(float (&o)[CPL], const float (&v)[CPL], int zb, in
     if (zb + c + 1 >= Z) o[c] = 0.0f;
 }
 
-// Cell z-1 for each of the lane's cells (0 below z = 0).
-template <int CPL>
+// Cell z-1 for each of the lane's cells (0 below z = 0); `lane` counts
+// within the column's W lanes.
+template <int CPL, int W = 64>
 __device__ inline void z_prev(float (&o)[CPL], const float (&v)[CPL], int lane) {
-  float dn = __shfl_up(v[CPL - 1], 1, 64);
+  float dn = __shfl_up(v[CPL - 1], 1, W);
   o[0] = lane == 0 ? 0.0f : dn;
 #pragma unroll
   for (int c = 1; c < CPL; ++c) o[c] = v[c - 1];
