@@ -73,12 +73,15 @@ def cpu_baseline(kw, target_s=10.0):
   bpc64 = 2.0 * algorithmic_bytes_per_cell(kw["pml_sigma"].shape[0], lo, hi)
   copy_gbs = host_copy_gbs(threads)
   achieved = cells * n * bpc64 / wall / 1e9
+  full = workloads.time_steps(kw)
   return {"value": round(cells * n / wall / 1e9, 5), "unit": "Gcell-updates/s",
-          "cores": threads, "kind": "port",
+          "cores": threads, "kind": "port", "extrapolated": True,
           "sample": f"FP64 C oracle (oracle/fdtd_ref.c, OpenMP) on the full "
                     f"{kw['epsilon'].shape[1]}x{kw['epsilon'].shape[2]}x"
-                    f"{kw['epsilon'].shape[3]} grid, {n} steps in {wall:.1f} s "
-                    f"(set-up of {t_setup:.1f} s excluded)",
+                    f"{kw['epsilon'].shape[3]} grid, {n} of the run's {full} steps in "
+                    f"{wall:.1f} s (set-up of {t_setup:.1f} s excluded); the rate per "
+                    f"cell-update is extrapolated to the whole run "
+                    f"({cells * full / (cells * n / wall) / 60:.0f} min on these cores)",
           "algorithmic_gbs": round(achieved, 2), "host_copy_gbs": round(copy_gbs, 2),
           "roofline_frac": round(achieved / copy_gbs, 4)}
 

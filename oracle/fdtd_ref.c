@@ -27,9 +27,14 @@ int oracle_fdtd_run(int X, int Y, int Z, int lo, int hi, double dt, const double
                     int out_stop, int out_step, int x0, int y0, int z0, int ex, int ey, int ez,
                     double* out, int nsteps_limit, int nthreads, double* E_final,
                     double* H_final) {
+  /* Thread count of this call only (num_threads clauses below): setting the
+     process-wide default would leave later calls with threads = 0 on the
+     count an earlier call asked for. */
+  int nt = 1;
 #ifdef _OPENMP
-  if (nthreads > 0) omp_set_num_threads(nthreads);
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #endif
+  (void)nt;
   const long N = (long)X * Y * Z;
   const int NP = lo + hi;
   double* E = calloc(3 * N, sizeof(double));
@@ -69,7 +74,7 @@ int oracle_fdtd_run(int X, int Y, int Z, int lo, int hi, double dt, const double
   (void)tt;
   double *Ex = E, *Ey = E + N, *Ez = E + 2 * N, *Hx = H, *Hy = H + N, *Hz = H + 2 * N;
   for (int n = 0; n < nsteps; ++n) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int i = 0; i < X; ++i)
       for (int j = 0; j < Y; ++j)
         for (int k = 0; k < Z; ++k) {
@@ -94,7 +99,7 @@ int oracle_fdtd_run(int X, int Y, int Z, int lo, int hi, double dt, const double
     int out_idx = -1;
     if (n >= out_start && n < out_stop && (n - out_start) % out_step == 0)
       out_idx = (n - out_start) / out_step;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int i = 0; i < X; ++i)
       for (int j = 0; j < Y; ++j)
         for (int k = 0; k < Z; ++k) {
