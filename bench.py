@@ -10,7 +10,11 @@ average launch duration, HIP events on the kernel's stream), and the FP64
 CPU oracle timed on a bounded sample of the same workload on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                       [--kernel auto|reference|stream|stream2] [--no-cpu-baseline]
+                       [--kernel auto|reference|stream|stream2|stream3|stream4]
+                       [--no-cpu-baseline]
+
+--workload native_512_z64 runs the same at 512x512x64, where the default
+kernel takes three steps per launch.
 
 Other workloads (one JSON line each): --workload grad_256 (forward +
 adjoint gradient, 256x256x128, 6000 steps), --workload batch64 (64 uneven
@@ -435,7 +439,7 @@ def main():
   ap.add_argument("--warmup", type=int, default=1)
   ap.add_argument("--workload", default="native_512")
   ap.add_argument("--kernel", default="auto",
-                  choices=["auto", "reference", "stream", "stream2"])
+                  choices=["auto", "reference", "stream", "stream2", "stream3", "stream4"])
   ap.add_argument("--time-steps", type=int, default=0,
                   help="override the FDTD step count (profiling runs only)")
   ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -488,12 +492,14 @@ def main():
   ms_per_step = wall * 1e3 / args.steps
   value = cells * tsteps * args.steps / wall / 1e9
 
-  kernel_name = "stream2" if args.kernel == "auto" else args.kernel
-  # Stepper launches per run: the two-step kernel advances two steps per
-  # launch (plus a one-step launch for an odd last step); the reference
-  # stepper's H + E launch pair counts as one launch of one step.
-  launches = (tsteps + 1) // 2 if kernel_name == "stream2" else tsteps
-  temporal_block = 2 if kernel_name == "stream2" else 1
+  kernel_name = resolved_stepper(kw)
+  # Stepper launches per run: a kernel of depth T advances T steps per
+  # launch; the remainder runs as one two-step and/or one one-step launch.
+  # The reference stepper's H + E launch pair counts as one launch of one
+  # step.
+  temporal_block = {"stream2": 2, "stream3": 3, "stream4": 4}.get(kernel_name, 1)
+  rem = tsteps % temporal_block
+  launches = tsteps // temporal_block + (rem // 2 + rem % 2)
   bpc = algorithmic_bytes_per_cell(zz, lo, hi)
   bytes_per_launch = cells * bpc
   avg_launch_s = (sum(gpu_ms) / len(gpu_ms)) / 1e3 / launches
@@ -530,6 +536,22 @@ def main():
     np_kw = builder(steps=args.time_steps) if args.time_steps else builder()
     line["cpu_baseline"] = cpu_baseline(np_kw)
   print(json.dumps(line), flush=True)
+
+
+def resolved_stepper(kw):
+  """Name of the stepper kernel fdtdz() runs for these arguments on this
+  device (the library resolves "auto" / "autotune")."""
+  from pjz_amd import _lib
+  from pjz_amd import fdtdz_torch as F
+  shape = {k: tuple(v.shape) for k, v in kw.items() if hasattr(v, "shape")}
+  prob = F.make_problem(shape["epsilon"], kw["dt"], shape["source_field"],
+                        kw["source_position"], shape["absorption_mask"], shape["pml_kappa"],
+                        kw["pml_widths"], kw["output_steps"], shape["source_waveform"][0],
+                        kw["use_reduced_precision"], kw["launch_params"], kw["offset"])
+  names = {_lib.KERNEL_REFERENCE: "reference", _lib.KERNEL_STREAM: "stream",
+           _lib.KERNEL_STREAM2: "stream2", _lib.KERNEL_STREAM3: "stream3",
+           _lib.KERNEL_STREAM4: "stream4"}
+  return names[_lib.stepper(prob)]
 
 
 def one_step_roofline(builder, dev, bytes_per_launch, cells, tsteps=1000):

@@ -55,9 +55,11 @@ extern "C" {
 #define PJZ_FDTD_KERNEL_REFERENCE 1 /* two launches per step, one thread/cell */
 #define PJZ_FDTD_KERNEL_STREAM 2    /* one fused launch per step, x-sweep    */
 #define PJZ_FDTD_KERNEL_STREAM2 3   /* two steps per launch (temporal block) */
-#define PJZ_FDTD_KERNEL_AUTOTUNE 4  /* first call per shape times the kernels and */
-                                    /* tilings above (synchronising the stream), */
-                                    /* later calls reuse the fastest             */
+#define PJZ_FDTD_KERNEL_AUTOTUNE 4  /* first call per shape times the kernels,   */
+                                    /* depths and tilings (synchronising the     */
+                                    /* stream), later calls reuse the fastest    */
+#define PJZ_FDTD_KERNEL_STREAM3 5   /* three steps per launch (zz <= 64)         */
+#define PJZ_FDTD_KERNEL_STREAM4 6   /* four steps per launch (zz <= 64)          */
 
 /* Problem description: the static arguments of fdtdz(...). */
 typedef struct pjz_fdtd_problem {
@@ -213,6 +215,14 @@ PJZ_FDTD_API int pjz_fdtd_mode_solve(int32_t ww, int32_t xx, int32_t yy, int32_t
                                      float* x, float* w, float* err, int32_t* iters,
                                      int32_t max_iters, float tol, void* workspace,
                                      size_t workspace_bytes, void* stream);
+
+/*
+ * Diagnostic: the stepper (PJZ_FDTD_KERNEL_REFERENCE .. STREAM4) a forward
+ * call with this problem runs on the current device: AUTO and AUTOTUNE
+ * resolved (AUTOTUNE to its cached choice, or AUTO before the first call).
+ * No counterpart in pjz: bench.py reports it beside the kernel's timings.
+ */
+PJZ_FDTD_API int pjz_fdtd_stepper(const pjz_fdtd_problem* p, int32_t* kernel);
 
 /*
  * Diagnostic: copies `bytes` (a multiple of 16; both pointers 16-byte

@@ -14,6 +14,7 @@ ABI_VERSION = 1
 
 OK, EINVAL, EHIP, ESPACE = 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_REFERENCE, KERNEL_STREAM, KERNEL_STREAM2, KERNEL_AUTOTUNE = 0, 1, 2, 3, 4
+KERNEL_STREAM3, KERNEL_STREAM4 = 5, 6
 
 # Every entry point include/pjz_fdtd.h declares.
 EXPORTS = (
@@ -28,6 +29,7 @@ EXPORTS = (
     "pjz_fdtd_batch_forward",
     "pjz_fdtd_batch_backward",
     "pjz_fdtd_copy",
+    "pjz_fdtd_stepper",
     "pjz_fdtd_mode_workspace_bytes",
     "pjz_fdtd_mode_solve",
 )
@@ -94,6 +96,8 @@ def load():
                                             [vp, sz, vp])
     lib.pjz_fdtd_copy.restype = ctypes.c_int
     lib.pjz_fdtd_copy.argtypes = [vp, vp, sz, vp]
+    lib.pjz_fdtd_stepper.restype = ctypes.c_int
+    lib.pjz_fdtd_stepper.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
     i32 = ctypes.c_int32
     lib.pjz_fdtd_mode_workspace_bytes.restype = ctypes.c_int
     lib.pjz_fdtd_mode_workspace_bytes.argtypes = [i32] * 4 + [ctypes.POINTER(sz)]
@@ -120,6 +124,14 @@ def workspace_bytes(prob):
   check(load().pjz_fdtd_workspace_bytes(ctypes.byref(prob), ctypes.byref(n)),
         "pjz_fdtd_workspace_bytes")
   return n.value
+
+
+def stepper(prob):
+  """The stepper kernel id a forward call with ``prob`` runs on the current
+  device (AUTO / AUTOTUNE resolved)."""
+  k = ctypes.c_int32(0)
+  check(load().pjz_fdtd_stepper(ctypes.byref(prob), ctypes.byref(k)), "pjz_fdtd_stepper")
+  return k.value
 
 
 def num_outputs(prob):

@@ -79,7 +79,7 @@ int check(const pjz_fdtd_problem* p) {
       return fail(PJZ_FDTD_EINVAL, "last output step %d needs waveform rows > %d (tt=%d)", last,
                   last, p->tt);
   }
-  if (p->kernel < PJZ_FDTD_KERNEL_AUTO || p->kernel > PJZ_FDTD_KERNEL_AUTOTUNE)
+  if (p->kernel < PJZ_FDTD_KERNEL_AUTO || p->kernel > PJZ_FDTD_KERNEL_STREAM4)
     return fail(PJZ_FDTD_EINVAL, "unknown kernel id %d", p->kernel);
   if (p->kernel == PJZ_FDTD_KERNEL_STREAM && p->zz > 256)
     return fail(PJZ_FDTD_EINVAL, "the stream kernel needs zz <= 256, got %d", p->zz);
@@ -87,6 +87,10 @@ int check(const pjz_fdtd_problem* p) {
     return fail(PJZ_FDTD_EINVAL,
                 "the two-step stream kernel needs zz <= 64, an even zz <= 128 or a zz <= 256 "
                 "divisible by 4, got %d", p->zz);
+  if (p->kernel == PJZ_FDTD_KERNEL_STREAM3 && !pjz::stream_deep_supported(geom_of(*p), 3))
+    return fail(PJZ_FDTD_EINVAL, "the three-step stream kernel needs zz <= 64, got %d", p->zz);
+  if (p->kernel == PJZ_FDTD_KERNEL_STREAM4 && !pjz::stream_deep_supported(geom_of(*p), 4))
+    return fail(PJZ_FDTD_EINVAL, "the four-step stream kernel needs zz <= 64, got %d", p->zz);
   return PJZ_FDTD_OK;
 }
 
@@ -186,13 +190,17 @@ hipError_t autotune(const pjz::Geom& g, const pjz_fdtd_problem& p, const float* 
       cands.push_back({PJZ_FDTD_KERNEL_STREAM2, 0, 8});
     }
   }
+  // Deeper temporal blocking: three or four steps per pass.
+  if (pjz::stream_deep_supported(g, 3)) cands.push_back({PJZ_FDTD_KERNEL_STREAM3, 0, 0});
+  if (pjz::stream_deep_supported(g, 4)) cands.push_back({PJZ_FDTD_KERNEL_STREAM4, 0, 0});
   if (g.Z <= 256) {
     cands.push_back({PJZ_FDTD_KERNEL_STREAM, 0, 0});
     if (cpl2) cands.push_back({PJZ_FDTD_KERNEL_STREAM, 0, 4});
   }
   cands.push_back({PJZ_FDTD_KERNEL_REFERENCE, 0, 0});
-  // Up to 8 steps, within the waveform's rows (the run's own step count).
-  const int nsteps = std::min(8, pjz::steps_of(p)) & ~1;
+  // Up to 12 steps (a whole number of launches at every depth), within the
+  // waveform's rows (the run's own step count).
+  const int nsteps = std::min(12, pjz::steps_of(p)) & ~1;
   if (nsteps < 2) {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tuned[tune_key(p)] = cands[0];
@@ -240,9 +248,17 @@ hipError_t autotune(const pjz::Geom& g, const pjz_fdtd_problem& p, const float* 
   return e;
 }
 
-// Steps [b, end) with the kernel the problem selects (AUTO: the two-step
-// kernel where supported, else the one-step kernel if zz <= 256; AUTOTUNE:
-// the cached choice for the shape, else AUTO).
+// The stepper AUTO runs: three steps per launch on large grids of zz <= 64,
+// else the two-step kernel where supported, else the one-step kernel if
+// zz <= 256, else the reference kernel.
+int auto_kernel(const pjz::Geom& g) {
+  if (pjz::stream_deep_preferred(g)) return PJZ_FDTD_KERNEL_STREAM3;
+  if (pjz::stream2_supported(g)) return PJZ_FDTD_KERNEL_STREAM2;
+  return g.Z <= 256 ? PJZ_FDTD_KERNEL_STREAM : PJZ_FDTD_KERNEL_REFERENCE;
+}
+
+// Steps [b, end) with the kernel the problem selects (AUTO: auto_kernel;
+// AUTOTUNE: the cached choice for the shape, else AUTO).
 hipError_t run_steps(const pjz::Geom& g, const pjz_fdtd_problem& p, const float* mask,
                      const pjz::Src& src, const pjz::OutWin& win, char* ws,
                      const pjz::Layout& L, int b, int end, int* cur, float* capture,
@@ -259,13 +275,14 @@ hipError_t run_steps(const pjz::Geom& g, const pjz_fdtd_problem& p, const float*
     }
     kernel = PJZ_FDTD_KERNEL_AUTO;
   }
-  if (kernel == PJZ_FDTD_KERNEL_AUTO)
-    kernel = pjz::stream2_supported(g) ? PJZ_FDTD_KERNEL_STREAM2
-             : g.Z <= 256             ? PJZ_FDTD_KERNEL_STREAM
-                                      : PJZ_FDTD_KERNEL_REFERENCE;
+  if (kernel == PJZ_FDTD_KERNEL_AUTO) kernel = auto_kernel(g);
   pjz::StepRange r{b, end, cur, capture};
   if (kernel == PJZ_FDTD_KERNEL_STREAM2)
     return pjz::run_stream2(g, p, mask, src, win, ws, L, r, s);
+  if (kernel == PJZ_FDTD_KERNEL_STREAM3)
+    return pjz::run_stream_deep(g, p, mask, src, win, ws, L, r, 3, s);
+  if (kernel == PJZ_FDTD_KERNEL_STREAM4)
+    return pjz::run_stream_deep(g, p, mask, src, win, ws, L, r, 4, s);
   if (kernel == PJZ_FDTD_KERNEL_STREAM) return pjz::run_stream(g, p, mask, src, win, ws, L, r, s);
   return pjz::run_reference(g, p, mask, src, win, ws, L, r, s);
 }
@@ -367,6 +384,18 @@ int pjz_fdtd_copy(const void* src, void* dst, size_t bytes, void* stream) {
   if (bytes == 0) return PJZ_FDTD_OK;
   hipError_t e = pjz::launch_copy_stream(dst, src, bytes / 16, (hipStream_t)stream);
   return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "copy");
+}
+
+int pjz_fdtd_stepper(const pjz_fdtd_problem* p, int32_t* kernel) {
+  int rc = check(p);
+  if (rc) return rc;
+  if (!kernel) return fail(PJZ_FDTD_EINVAL, "kernel is NULL");
+  const pjz::Geom g = geom_of(*p);
+  int k = p->kernel;
+  Tuned t;
+  if (k == PJZ_FDTD_KERNEL_AUTOTUNE) k = tuned_choice(*p, &t) ? t.kernel : PJZ_FDTD_KERNEL_AUTO;
+  *kernel = k == PJZ_FDTD_KERNEL_AUTO ? auto_kernel(g) : k;
+  return PJZ_FDTD_OK;
 }
 
 int pjz_fdtd_num_outputs(const pjz_fdtd_problem* p, int32_t* nn) {

@@ -165,6 +165,15 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
                        const Src& src, const OutWin& win, char* ws, const Layout& L,
                        const StepRange& r, hipStream_t s);
 
+// T = 3 or 4 steps per launch (deeper temporal blocking, zz <= 64); fewer
+// than T remaining steps run on run_stream2.
+bool stream_deep_supported(const Geom& g, int T);
+// Whether the default (AUTO) kernel choice is three steps per launch.
+bool stream_deep_preferred(const Geom& g);
+hipError_t run_stream_deep(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
+                           const Src& src, const OutWin& win, char* ws, const Layout& L,
+                           const StepRange& r, int T, hipStream_t s);
+
 // One simulation of a batch: its problem, arrays and workspace.  `capture`
 // (or null) receives E of the whole domain after each step, as in StepRange,
 // counted from the step set by batch2_capture_from.

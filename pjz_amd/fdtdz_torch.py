@@ -21,6 +21,7 @@ from . import _lib
 
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "reference": _lib.KERNEL_REFERENCE,
             "stream": _lib.KERNEL_STREAM, "stream2": _lib.KERNEL_STREAM2,
+            "stream3": _lib.KERNEL_STREAM3, "stream4": _lib.KERNEL_STREAM4,
             "autotune": _lib.KERNEL_AUTOTUNE}
 _LAUNCH_KEYS = {"kernel", "tile_x", "tile_y", "waves_per_simd", "ckpt_every", "graph_steps"}
 
@@ -28,8 +29,9 @@ _LAUNCH_KEYS = {"kernel", "tile_x", "tile_y", "waves_per_simd", "ckpt_every", "g
 def _launch_options(launch_params):
   """Parses ``launch_params``: None, a mapping, or a tuple of (key, value).
 
-  Keys: ``kernel`` ("auto" | "reference" | "stream" | "stream2" | "autotune": the
-  first call per shape times the kernels and tilings, later calls reuse the
+  Keys: ``kernel`` ("auto" | "reference" | "stream" | "stream2" | "stream3" |
+  "stream4" (one, two, three or four steps per pass) | "autotune": the first
+  call per shape times the kernels, depths and tilings, later calls reuse the
   fastest), ``tile_x``, ``tile_y``,
   ``waves_per_simd`` (stream kernel tiling), ``ckpt_every`` (steps between
   checkpoints kept for the gradient; default about sqrt(2 * steps)),

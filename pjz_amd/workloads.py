@@ -51,6 +51,12 @@ def layered_64(steps=2000, seed=0):
       launch_params=None, offset=(0, 0, 0))
 
 
+def native_512_z64(steps=10000, seed=0):
+  """native_512 at half height, 512x512x64 (one cell per lane: the height
+  where three steps per launch pay)."""
+  return native_512(steps=steps, seed=seed, zz=64)
+
+
 def native_512(steps=10000, seed=0, xx=512, yy=512, zz=128, design=200):
   """The 512x512x128 workload described above; ``xx``, ``yy``, ``zz`` and
   the ``design`` square's side shrink it for parity runs the oracle can

@@ -69,6 +69,20 @@ def test_bad_problem_raises_value_error(over, msg):
     _lib.workspace_bytes(_problem(**over))
 
 
+def test_deep_kernel_ids_checked():
+  """Three and four steps per launch: accepted at zz = 64, refused at
+  zz = 96 (one cell per lane only)."""
+  for kernel in ("stream3", "stream4"):
+    assert _lib.workspace_bytes(_problem(launch_params=dict(kernel=kernel))) > 0
+  z96 = dict(epsilon=np.zeros((3, 64, 64, 96), np.float32),
+             source_field=np.zeros((2, 1, 64, 96), np.float32),
+             pml_kappa=np.ones((96, 2), np.float32))
+  with pytest.raises(ValueError, match="three-step"):
+    _lib.workspace_bytes(_problem(launch_params=dict(kernel="stream3"), **z96))
+  with pytest.raises(ValueError, match="four-step"):
+    _lib.workspace_bytes(_problem(launch_params=dict(kernel="stream4"), **z96))
+
+
 def test_bad_shapes_raise_value_error():
   with pytest.raises(ValueError, match="source_field"):
     F.source_geometry((2, 3, 64, 64), (64, 64, 64))
