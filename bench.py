@@ -21,7 +21,9 @@ adjoint gradient, 256x256x128, 6000 steps), --workload batch64 (64 uneven
 simulations batched against 64 sequential calls) and --workload
 design_batch (64 simulations of 128x128x64, forward + gradient, batched
 against sequential calls) and --workload design_batch_384 (the same at the
-full 384x384x128, 8000 steps, in memory-bounded waves).
+full 384x384x128, 8000 steps, in memory-bounded waves), --workload mode (the
+mode solver kernel) and --workload density (pjz's density() on its HIP
+kernels, forward + gradient).
 """
 
 import argparse
@@ -432,6 +434,70 @@ def bench_mode(args):
   print(json.dumps(line), flush=True)
 
 
+def bench_density(args):
+  """pjz's density() on the HIP kernels: forward + reverse pass of a
+  4096 x 4096 design variable (cone radius 4: 81 taps), against the torch
+  restatement of the same function on the same GPU (float32), with the blur
+  kernel's arithmetic rate beside the FP32 vector peak."""
+  import numpy as np
+  import torch
+  from pjz_amd import _density as D
+  dev = torch.device("cuda", 0)
+  X = Y = 4096
+  radius, alpha = 4.0, 0.6
+  reps = args.steps
+  u = torch.rand((X, Y), device=dev)
+  gd, gl = torch.randn((X, Y), device=dev), torch.randn((X, Y), device=dev)
+
+  def hip():
+    v = u.clone().requires_grad_(True)
+    dens, loss = D.density(v, radius, alpha)
+    ((dens * gd).sum() + (loss * gl).sum()).backward()
+
+  def restated():
+    v = u.clone().requires_grad_(True)
+    uf = D.blur(v, radius)
+    up = D.project(uf, 0.5)
+    dens, loss = alpha * up + (1 - alpha) * uf, D.geom_loss(up, uf, 1.0, 0.25, 0.75)
+    ((dens * gd).sum() + (loss * gl).sum()).backward()
+
+  times = {}
+  for name, fn in (("hip", hip), ("torch", restated)):
+    for _ in range(max(args.warmup, 1)):
+      fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+      fn()
+    torch.cuda.synchronize()
+    times[name] = (time.perf_counter() - t0) / reps
+  # The forward alone (blur + projection kernels), events on the current
+  # stream; the blur's multiply-adds over that time bound its rate below.
+  v = u.clone()
+  taps = D.cone(radius).shape[0] ** 2
+  start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  start.record()
+  for _ in range(reps):
+    D.density(v, radius, alpha)
+  end.record()
+  torch.cuda.synchronize()
+  fwd_s = start.elapsed_time(end) / 1e3 / reps
+  line = {"metric": "density() forward + gradient, pixels per second", "unit": "Mpixel/s",
+          "value": round(X * Y / times["hip"] / 1e6, 1), "n_gpus": 1, "steps": reps,
+          "warmup": args.warmup, "ms_per_step": round(times["hip"] * 1e3, 3),
+          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+          "data": "synthetic",
+          "config": {"workload": f"density: {X}x{Y} variable, cone radius {radius} "
+                                 f"({taps} taps), forward + reverse pass",
+                     "hip_ms": round(times["hip"] * 1e3, 3),
+                     "torch_restated_ms": round(times["torch"] * 1e3, 3),
+                     "speedup_vs_torch": round(times["torch"] / times["hip"], 2),
+                     "forward_ms": round(fwd_s * 1e3, 3),
+                     "forward_blur_tflops_bound": round(2 * taps * X * Y / fwd_s / 1e12, 2),
+                     "fp32_vector_peak_tflops": 157.3}}
+  print(json.dumps(line), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -458,6 +524,8 @@ def main():
     return bench_design_full(args)
   if args.workload == "mode":
     return bench_mode(args)
+  if args.workload == "density":
+    return bench_density(args)
 
   import numpy as np
   import torch

@@ -217,6 +217,31 @@ PJZ_FDTD_API int pjz_fdtd_mode_solve(int32_t ww, int32_t xx, int32_t yy, int32_t
                                      size_t workspace_bytes, void* stream);
 
 /*
+ * pjz's density() (src/pjz/_density.py:48-79) for nb design variables of
+ * (xx, yy), fp32 device arrays, y fastest: the cone blur of radius `radius`
+ * (half-width ceil(radius - 1/2) <= PJZ_FDTD_DENSITY_MAX_HALF_WIDTH) of the
+ * edge-padded variable (`_filter`, :17-20), the eta projection (`_project`,
+ * :30-31) and the minimum-feature-size loss (`_geom_loss`, :41-45).
+ * Forward writes density, loss and the blurred variable `ufilt`, which the
+ * backward takes back to compute g_u = dL/du from the cotangents of density
+ * and loss (the vector-Jacobian product jax.grad forms through density()),
+ * with a workspace of pjz_fdtd_density_workspace_bytes (256-byte aligned).
+ */
+#define PJZ_FDTD_DENSITY_MAX_HALF_WIDTH 40
+PJZ_FDTD_API int pjz_fdtd_density_workspace_bytes(int32_t nb, int32_t xx, int32_t yy,
+                                                  float radius, size_t* bytes);
+PJZ_FDTD_API int pjz_fdtd_density_forward(int32_t nb, int32_t xx, int32_t yy, float radius,
+                                          float alpha, float c, float eta, float eta_lo,
+                                          float eta_hi, const float* u, float* density,
+                                          float* loss, float* ufilt, void* stream);
+PJZ_FDTD_API int pjz_fdtd_density_backward(int32_t nb, int32_t xx, int32_t yy, float radius,
+                                           float alpha, float c, float eta, float eta_lo,
+                                           float eta_hi, const float* ufilt,
+                                           const float* g_density, const float* g_loss,
+                                           float* g_u, void* workspace, size_t workspace_bytes,
+                                           void* stream);
+
+/*
  * Diagnostic: the stepper (PJZ_FDTD_KERNEL_REFERENCE .. STREAM4) a forward
  * call with this problem runs on the current device: AUTO and AUTOTUNE
  * resolved (AUTOTUNE to its cached choice, or AUTO before the first call).

@@ -6,7 +6,9 @@ kernel ``max(0, r - distance)`` normalised, ``:9-14``), a projection that
 binarises every pixel except those on the ``eta`` boundary (``_project``,
 ``:23-31``), and a minimum-feature-size loss weighted by an inflection
 indicator ``exp(-c |grad|^2)`` (``_inflection`` / ``_geom_loss``,
-``:34-45``).  torch, on the variable's device, differentiable.
+``:34-45``).  Differentiable.  A float32 variable on the GPU runs the HIP
+kernels of ``csrc/kernels_density.hip`` (forward and exact reverse pass); a
+CPU variable runs the torch restatement below on the host.
 """
 
 import math
@@ -65,14 +67,64 @@ def geom_loss(uproj, ufilt, c, eta_lo, eta_hi):
           (1 - uproj) * w * torch.minimum(zero, eta_lo - ufilt) ** 2)
 
 
+class _DensityKernel(torch.autograd.Function):
+  """density() through the C ABI (``pjz_fdtd_density_forward`` /
+  ``_backward``) for a float32 device variable."""
+
+  @staticmethod
+  def forward(ctx, u, radius, alpha, c, eta, eta_lo, eta_hi):
+    from . import _lib
+    u = u.detach().contiguous()
+    X, Y = u.shape
+    dens, loss, uf = (torch.empty_like(u) for _ in range(3))
+    with torch.cuda.device(u.device):
+      stream = torch.cuda.current_stream(u.device).cuda_stream
+      _lib.check(_lib.load().pjz_fdtd_density_forward(
+          1, X, Y, radius, alpha, c, eta, eta_lo, eta_hi, u.data_ptr(), dens.data_ptr(),
+          loss.data_ptr(), uf.data_ptr(), stream), "pjz_fdtd_density_forward")
+    ctx.save_for_backward(uf)
+    ctx.args = (radius, alpha, c, eta, eta_lo, eta_hi)
+    return dens, loss
+
+  @staticmethod
+  def backward(ctx, g_dens, g_loss):
+    from . import _lib
+    import ctypes
+    (uf,) = ctx.saved_tensors
+    X, Y = uf.shape
+    g_dens = torch.zeros_like(uf) if g_dens is None else g_dens.to(uf.dtype).contiguous()
+    g_loss = torch.zeros_like(uf) if g_loss is None else g_loss.to(uf.dtype).contiguous()
+    radius = ctx.args[0]
+    lib = _lib.load()
+    nbytes = ctypes.c_size_t(0)
+    _lib.check(lib.pjz_fdtd_density_workspace_bytes(1, X, Y, radius, ctypes.byref(nbytes)),
+               "pjz_fdtd_density_workspace_bytes")
+    # torch's caching allocator hands out 512-byte aligned blocks.
+    ws = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=uf.device)
+    g_u = torch.empty_like(uf)
+    with torch.cuda.device(uf.device):
+      stream = torch.cuda.current_stream(uf.device).cuda_stream
+      _lib.check(lib.pjz_fdtd_density_backward(
+          1, X, Y, *ctx.args, uf.data_ptr(), g_dens.data_ptr(), g_loss.data_ptr(),
+          g_u.data_ptr(), ws.data_ptr(), nbytes.value, stream), "pjz_fdtd_density_backward")
+    return g_u, None, None, None, None, None, None
+
+
 def density(u, radius, alpha, c=1.0, eta=0.5, eta_lo=0.25, eta_hi=0.75):
   """pjz's ``density()`` (``_density.py:48-79``): ``(density, loss)``, both
-  shaped like the (xx, yy) variable ``u``."""
+  shaped like the (xx, yy) variable ``u``.  A GPU variable runs the HIP
+  kernels (float32 only: they compute in float32); a CPU variable runs the
+  torch restatement on the host."""
   u = torch.as_tensor(u)
   if not u.is_floating_point():
     u = u.to(torch.float32)
   if u.ndim != 2:
     raise ValueError(f"u must be (xx, yy), got {tuple(u.shape)}")
+  if u.is_cuda:
+    if u.dtype != torch.float32:
+      raise ValueError(f"density() on the GPU computes in float32, got {u.dtype}")
+    return _DensityKernel.apply(u, float(radius), float(alpha), float(c), float(eta),
+                                float(eta_lo), float(eta_hi))
   ufilt = blur(u, radius)
   uproj = project(ufilt, eta)
   return alpha * uproj + (1 - alpha) * ufilt, geom_loss(uproj, ufilt, c, eta_lo, eta_hi)

@@ -30,6 +30,9 @@ EXPORTS = (
     "pjz_fdtd_batch_backward",
     "pjz_fdtd_copy",
     "pjz_fdtd_stepper",
+    "pjz_fdtd_density_workspace_bytes",
+    "pjz_fdtd_density_forward",
+    "pjz_fdtd_density_backward",
     "pjz_fdtd_mode_workspace_bytes",
     "pjz_fdtd_mode_solve",
 )
@@ -103,6 +106,13 @@ def load():
     lib.pjz_fdtd_mode_workspace_bytes.argtypes = [i32] * 4 + [ctypes.POINTER(sz)]
     lib.pjz_fdtd_mode_solve.restype = ctypes.c_int
     lib.pjz_fdtd_mode_solve.argtypes = [i32] * 4 + [vp] * 7 + [i32, ctypes.c_float, vp, sz, vp]
+    f32 = ctypes.c_float
+    lib.pjz_fdtd_density_workspace_bytes.restype = ctypes.c_int
+    lib.pjz_fdtd_density_workspace_bytes.argtypes = [i32] * 3 + [f32, ctypes.POINTER(sz)]
+    lib.pjz_fdtd_density_forward.restype = ctypes.c_int
+    lib.pjz_fdtd_density_forward.argtypes = [i32] * 3 + [f32] * 6 + [vp] * 4 + [vp]
+    lib.pjz_fdtd_density_backward.restype = ctypes.c_int
+    lib.pjz_fdtd_density_backward.argtypes = [i32] * 3 + [f32] * 6 + [vp] * 4 + [vp, sz, vp]
     if lib.pjz_fdtd_abi_version() != ABI_VERSION:
       raise RuntimeError("libpjzfdtd.so ABI version mismatch; rebuild it")
     _lib = lib

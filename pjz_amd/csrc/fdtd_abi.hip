@@ -386,6 +386,64 @@ int pjz_fdtd_copy(const void* src, void* dst, size_t bytes, void* stream) {
   return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "copy");
 }
 
+namespace {
+// Arguments of the density entry points.
+int check_density(int32_t nb, int32_t xx, int32_t yy, float radius, float eta_lo,
+                  float eta_hi) {
+  if (nb < 1 || xx < 1 || yy < 1)
+    return fail(PJZ_FDTD_EINVAL, "density: nb, xx, yy must be >= 1, got %d, %d, %d", nb, xx, yy);
+  if ((long long)nb * xx * yy > (1LL << 31) / 4)
+    return fail(PJZ_FDTD_EINVAL, "density: nb * xx * yy too large");
+  if (!(radius > 0.0f) || pjz::density_half_width(radius) > PJZ_FDTD_DENSITY_MAX_HALF_WIDTH)
+    return fail(PJZ_FDTD_EINVAL, "density: radius must be in (0, %d.5), got %g",
+                PJZ_FDTD_DENSITY_MAX_HALF_WIDTH, (double)radius);
+  if (!(eta_lo <= eta_hi))
+    return fail(PJZ_FDTD_EINVAL, "density: eta_lo must be <= eta_hi");
+  return PJZ_FDTD_OK;
+}
+}  // namespace
+
+int pjz_fdtd_density_workspace_bytes(int32_t nb, int32_t xx, int32_t yy, float radius,
+                                     size_t* bytes) {
+  int rc = check_density(nb, xx, yy, radius, 0.0f, 0.0f);
+  if (rc) return rc;
+  if (!bytes) return fail(PJZ_FDTD_EINVAL, "bytes is NULL");
+  *bytes = pjz::density_workspace_bytes(nb, xx, yy, radius);
+  return PJZ_FDTD_OK;
+}
+
+int pjz_fdtd_density_forward(int32_t nb, int32_t xx, int32_t yy, float radius, float alpha,
+                             float c, float eta, float eta_lo, float eta_hi, const float* u,
+                             float* density, float* loss, float* ufilt, void* stream) {
+  int rc = check_density(nb, xx, yy, radius, eta_lo, eta_hi);
+  if (rc) return rc;
+  if (!u || !density || !loss || !ufilt)
+    return fail(PJZ_FDTD_EINVAL, "density: a required pointer argument is NULL");
+  hipError_t e = pjz::density_forward(nb, xx, yy, radius, alpha, c, eta, eta_lo, eta_hi, u,
+                                      density, loss, ufilt, (hipStream_t)stream);
+  return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "density forward");
+}
+
+int pjz_fdtd_density_backward(int32_t nb, int32_t xx, int32_t yy, float radius, float alpha,
+                              float c, float eta, float eta_lo, float eta_hi, const float* ufilt,
+                              const float* g_density, const float* g_loss, float* g_u,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_density(nb, xx, yy, radius, eta_lo, eta_hi);
+  if (rc) return rc;
+  if (!ufilt || !g_density || !g_loss || !g_u || !workspace)
+    return fail(PJZ_FDTD_EINVAL, "density: a required pointer argument is NULL");
+  const size_t need = pjz::density_workspace_bytes(nb, xx, yy, radius);
+  if (workspace_bytes < need)
+    return fail(PJZ_FDTD_ESPACE, "density: workspace of %zu bytes, %zu needed", workspace_bytes,
+                need);
+  if (((uintptr_t)workspace & 255) != 0)
+    return fail(PJZ_FDTD_EINVAL, "density: workspace must be 256-byte aligned");
+  hipError_t e = pjz::density_backward(nb, xx, yy, radius, alpha, c, eta, eta_lo, eta_hi, ufilt,
+                                       g_density, g_loss, g_u, (char*)workspace,
+                                       (hipStream_t)stream);
+  return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "density backward");
+}
+
 int pjz_fdtd_stepper(const pjz_fdtd_problem* p, int32_t* kernel) {
   int rc = check(p);
   if (rc) return rc;

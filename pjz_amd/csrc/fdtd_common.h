@@ -265,6 +265,19 @@ hipError_t adjoint2_batch_begin_segment(const Adj2BatchPlan& plan, hipStream_t s
 hipError_t launch_adjoint2_batch_step(const Adj2BatchPlan& plan, int launch, int step,
                                       int seg_begin, hipStream_t s);
 
+// pjz's density() on nb (X, Y) variables (kernels_density.hip): the cone's
+// half-width r = ceil(radius - 1/2); forward writes density, loss and the
+// blurred variable uf (the backward's input); backward needs a workspace of
+// density_workspace_bytes.
+int density_half_width(float radius);
+size_t density_workspace_bytes(int nb, int X, int Y, float radius);
+hipError_t density_forward(int nb, int X, int Y, float radius, float alpha, float c, float eta,
+                           float lo, float hi, const float* u, float* dens, float* loss,
+                           float* uf, hipStream_t s);
+hipError_t density_backward(int nb, int X, int Y, float radius, float alpha, float c, float eta,
+                            float lo, float hi, const float* uf, const float* gd,
+                            const float* gl, float* gu, char* ws, hipStream_t s);
+
 // lE[window] += g (one snapshot cotangent, (3, ex, ey, ez)).
 hipError_t launch_inject(const Geom& g, const pjz_fdtd_problem& p, float* lE, const float* g_out,
                          hipStream_t s);

@@ -15,6 +15,7 @@ import math
 
 import numpy as np
 import pytest
+import scipy.ndimage
 import scipy.signal
 import torch
 
@@ -113,16 +114,80 @@ def test_differentiable():
 
 @pytest.mark.gpu
 def test_density_on_the_gpu(gpu):
+  """density() on the device against references independent of it: the blur
+  by scipy.signal.convolve on the edge-padded variable, boundary cells,
+  inflection and loss by the per-pixel loops above, all in float64."""
   rng = np.random.default_rng(4)
   u = rng.random((64, 48))
-  d0, l0 = D.density(torch.tensor(u), 3.5, 0.6)
-  d1, l1 = D.density(torch.tensor(u, dtype=torch.float32, device=gpu), 3.5, 0.6)
+  radius, alpha, c = 3.5, 0.6, 1.0
+  d1, l1 = D.density(torch.tensor(u, dtype=torch.float32, device=gpu), radius, alpha, c=c)
   assert d1.device.type == "cuda"
+  k = D.cone(radius, torch.float64).numpy()
+  r = k.shape[0] // 2
+  uf = scipy.signal.convolve(np.pad(u, r, mode="edge"), k, mode="valid")
+  bnd = loop_boundary(uf - 0.5)
+  proj = np.where(bnd, uf, (uf > 0.5).astype(float) + 0.5 * (uf == 0.5))
+  infl = loop_inflection(uf, c)
+  lo, hi = np.minimum(0.0, uf - 0.75), np.minimum(0.0, 0.25 - uf)
+  loss = infl * (proj * lo * lo + (1 - proj) * hi * hi)
   # Pixels whose blurred value, or a neighbour's, sits within float32
   # rounding of eta can binarise differently; compare everything else.
-  uf = D.blur(torch.tensor(u), 3.5).numpy()
   near = np.abs(uf - 0.5) <= 1e-5
   ok = ~(near | np.roll(near, 1, 0) | np.roll(near, -1, 0) | np.roll(near, 1, 1) |
          np.roll(near, -1, 1))
-  np.testing.assert_allclose(d1.cpu().numpy()[ok], d0.numpy()[ok], rtol=1e-5, atol=1e-6)
-  np.testing.assert_allclose(l1.cpu().numpy()[ok], l0.numpy()[ok], rtol=1e-4, atol=1e-7)
+  assert ok.mean() > 0.95
+  np.testing.assert_allclose(d1.cpu().numpy()[ok], (alpha * proj + (1 - alpha) * uf)[ok],
+                             rtol=1e-5, atol=1e-6)
+  np.testing.assert_allclose(l1.cpu().numpy()[ok], loss[ok], rtol=1e-4, atol=1e-7)
+
+
+def _near_threshold(uf, eta=0.5, tol=1e-5):
+  """Pixels whose blurred value, or a 4-neighbour's, is within float32
+  rounding of eta (their projection branch may differ between precisions)."""
+  near = np.abs(uf - eta) <= tol
+  return near | np.roll(near, 1, 0) | np.roll(near, -1, 0) | np.roll(near, 1, 1) | \
+      np.roll(near, -1, 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,radius", [((64, 48), 3.5), ((37, 29), 1.0), ((20, 33), 2.5),
+                                          ((40, 40), 6.0), ((1, 17), 2.0), ((2, 9), 3.0)])
+def test_density_kernel_gradient(gpu, shape, radius):
+  """The HIP reverse pass against torch.autograd through the float64 torch
+  restatement on the CPU (whose forward the tests above pin), for random
+  cotangents of both outputs; pixels near the threshold excluded."""
+  rng = np.random.default_rng(int(10 * radius) + shape[0])
+  u = rng.random(shape)
+  alpha, c = 0.7, 2.0
+  gd, gl = rng.standard_normal(shape), rng.standard_normal(shape)
+  uc = torch.tensor(u, requires_grad=True)
+  d0, l0 = D.density(uc, radius, alpha, c=c)
+  (d0 * torch.tensor(gd)).sum().backward(retain_graph=True)
+  gu_d = uc.grad.clone()
+  uc.grad = None
+  (l0 * torch.tensor(gl)).sum().backward()
+  ref = gu_d.numpy() + uc.grad.numpy()
+  ug = torch.tensor(u, dtype=torch.float32, device=gpu, requires_grad=True)
+  d1, l1 = D.density(ug, radius, alpha, c=c)
+  ((d1 * torch.tensor(gd, dtype=torch.float32, device=gpu)).sum() +
+   (l1 * torch.tensor(gl, dtype=torch.float32, device=gpu)).sum()).backward()
+  got = ug.grad.cpu().numpy()
+  uf = D.blur(torch.tensor(u), radius).numpy()
+  bad = _near_threshold(uf)
+  # A near-threshold pixel's branch feeds the gradient of every pixel its
+  # blur reads: compare where no such pixel lies within the cone.
+  r = D.cone(radius, torch.float64).shape[0] // 2 + 1
+  far = ~(scipy.ndimage.maximum_filter(bad.astype(np.uint8), size=2 * r + 1) > 0)
+  assert far.mean() > 0.5
+  err = np.linalg.norm((got - ref)[far]) / np.linalg.norm(ref[far])
+  print(f"density gradient {shape} r={radius}: rel L2 {err:.2e}", flush=True)
+  assert err <= 1e-5
+
+
+@pytest.mark.gpu
+def test_density_kernel_rejects_bad_radius(gpu):
+  u = torch.rand((8, 8), device=gpu)
+  with pytest.raises(ValueError, match="radius"):
+    D.density(u, 41.0, 0.5)
+  with pytest.raises(ValueError, match="float32"):
+    D.density(u.double(), 2.0, 0.5)
