@@ -687,8 +687,8 @@ int pjz_fdtd_batch_forward(const pjz_fdtd_problem* probs, int n, const float* co
   if (rc) return rc;
   char* base = (char*)workspace;
   pjz::Batch2Plan plan;
-  hipError_t e = pjz::batch2_prepare(items.data(), n, base + bp.meta_off, plan, s);
-  if (e == hipSuccess && bp.S == 0) e = pjz::batch2_pairs(plan, 0, plan.npairs, s);
+  hipError_t e = pjz::batch2_prepare(items.data(), n, base + bp.meta_off, plan, s, bp.S == 0);
+  if (e == hipSuccess && bp.S == 0) e = pjz::batch2_forward(plan, s);
   // With checkpoints: before the pairs of segment k, save the state of step
   // k*S (buffer set (k*S/2) & 1) of every simulation that reaches it.
   for (int k = 0; bp.S > 0 && e == hipSuccess && k * bp.S < 2 * plan.npairs; ++k) {

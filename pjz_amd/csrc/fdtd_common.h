@@ -138,8 +138,9 @@ hipError_t launch_copy(float* dst, const float* src, size_t n, hipStream_t s);
 hipError_t launch_copy_stream(void* dst, const void* src, size_t n16, hipStream_t s);
 
 // x chunk length for a batch class: minimises rounds x planes per tile over
-// candidate lengths <= max_lx (xs, nty: each simulation's X and row tiles).
-int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx);
+// candidate lengths <= max_lx (xs, nty: each simulation's X and row tiles;
+// `ghost`: recomputed planes per tile).
+int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx, int ghost = 2);
 int grid_chunk_x(int X, int nty, int slots, int ghost);
 
 // Waveguide mode solver (kernels_mode.hip): ww frequencies of a (xx, yy)
@@ -200,14 +201,20 @@ struct Batch2Plan {
   hipGraphExec_t exec;
   int count[2], base[2], total[2];  // per cells-per-lane class: sims, first sim, tiles
   int wy[2];                        // per class: rows per tile
+  int depth0, nlaunch0;  // zz <= 64 class: steps per launch (2, or 3 in
+                         // forward-only batches) and its three-step launches
   char* meta;
   size_t args_bytes, rare_bytes, ts_bytes;
 };
 bool batch2_supported(const Geom& g);
 size_t batch2_meta_bytes(const BatchItem* items, int n);
 hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan& plan,
-                          hipStream_t s);
+                          hipStream_t s, bool forward_only = false);
 hipError_t batch2_pairs(Batch2Plan& plan, int m0, int m1, hipStream_t s);
+// Every step of a forward-only plan (batch2_prepare with forward_only): the
+// zz <= 64 class may run three steps per launch; the final state is left in
+// either buffer set (only snapshots are read).
+hipError_t batch2_forward(Batch2Plan& plan, hipStream_t s);
 // Frees the plan's graph (synchronising the stream if one was built).
 hipError_t batch2_release(Batch2Plan& plan, hipStream_t s);
 hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s);

@@ -459,9 +459,10 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2_batch(
     const int* __restrict__ lstart, int nsims, int total, int* __restrict__ counter, int launch,
     int step, int seg_begin) {
   __shared__ Adj2Lds<CPL, WY> L;
-  __shared__ int s_item;
+  __shared__ int s_item, s_next;
+  if (threadIdx.x == 0) s_next = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    if (threadIdx.x == 0) s_item = pop_item(counter, total, s_next);
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
@@ -579,12 +580,12 @@ hipError_t launch_adjoint_stream2(const Geom& g, const pjz_fdtd_problem& p, cons
 // ---------------------------------------------------------------- batches
 // Meta layout: two argument views per simulation (cells-per-lane class
 // order) | tile prefix sums (n_c + 1 per class) | lstart (class order) |
-// queue counters (2 per launch slot of a segment).
+// queue counters (2 x kQueues per launch slot of a segment).
 static constexpr int kAdj2BatchLX = 128;
 
 size_t adjoint2_batch_meta_bytes(int n, int S) {
   return align256(sizeof(Adj2Args) * 2 * (size_t)n) + align256(sizeof(int) * (size_t)(n + 2)) +
-         align256(sizeof(int) * (size_t)n) + align256(sizeof(int) * 2 * (size_t)std::max(S, 1));
+         align256(sizeof(int) * (size_t)n) + align256(sizeof(int) * 2 * kQueues * (size_t)std::max(S, 1));
 }
 
 hipError_t adjoint2_batch_prepare(const BatchItem* items, int n, int S, const int* lstart,
@@ -641,7 +642,7 @@ hipError_t adjoint2_batch_prepare(const BatchItem* items, int n, int S, const in
 
 hipError_t adjoint2_batch_begin_segment(const Adj2BatchPlan& plan, hipStream_t s) {
   int* counters = (int*)(plan.meta + plan.args_bytes + plan.ts_bytes + plan.ls_bytes);
-  return launch_zero((float*)counters, 2 * (size_t)plan.S, s);
+  return launch_zero((float*)counters, 2 * kQueues * (size_t)plan.S, s);
 }
 
 hipError_t launch_adjoint2_batch_step(const Adj2BatchPlan& plan, int launch, int step,
@@ -661,7 +662,7 @@ hipError_t launch_adjoint2_batch_step(const Adj2BatchPlan& plan, int launch, int
     const Adj2Args* a = dargs + 2 * plan.base[c];
     const int* ts = dts + plan.base[c] + c;
     const int* ls = dls + plan.base[c];
-    int* ctr = counters + 2 * slot + c;
+    int* ctr = counters + (2 * slot + c) * kQueues;
     if (c == 0)
       hipLaunchKernelGGL((k_adj_stream2_batch<1, WY>), dim3(grid), dim3(64 * (WY + 3)), 0, s, a,
                          ts, ls, nc, plan.total[c], ctr, launch, step, seg_begin);

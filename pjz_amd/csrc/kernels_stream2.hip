@@ -329,12 +329,13 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_batch(
     const Stream2Args* __restrict__ args, const int* __restrict__ tile_start, int nsims,
     int total, int* __restrict__ counter, const int* __restrict__ step_base, int step_off) {
   __shared__ Lds2<CPL, WY> L;
-  __shared__ int s_item;
+  __shared__ int s_item, s_next;
+  if (threadIdx.x == 0) s_next = 0;
   // First step of the pair: step_off, plus a device-held base when the launch
   // is a node of a replayed graph chunk.
   const int step = (step_base ? uniform_load_int(step_base) : 0) + step_off;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    if (threadIdx.x == 0) s_item = pop_item(counter, total, s_next);
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
@@ -560,14 +561,15 @@ static int batch2_resident() {
 // on `slots` workgroups for each candidate LX and keep the longest chunk
 // within 3% of the shortest makespan (longer chunks re-read fewer ghost
 // planes, which the model does not charge).
-static long long batch_makespan(const int* xs, const int* nty, int n, int slots, int lx) {
+static long long batch_makespan(const int* xs, const int* nty, int n, int slots, int lx,
+                                int ghost) {
   std::priority_queue<long long, std::vector<long long>, std::greater<long long>> free_at;
   for (int w = 0; w < slots; ++w) free_at.push(0);
   long long span = 0;
   for (int i = 0; i < n; ++i) {
     const int ntx = (xs[i] + lx - 1) / lx;
     for (int tx = 0; tx < ntx; ++tx) {
-      const long long d = std::min(lx, xs[i] - tx * lx) + 2;
+      const long long d = std::min(lx, xs[i] - tx * lx) + ghost;
       for (int ty = 0; ty < nty[i]; ++ty) {
         const long long t = free_at.top() + d;
         free_at.pop();
@@ -579,7 +581,7 @@ static long long batch_makespan(const int* xs, const int* nty, int n, int slots,
   return span;
 }
 
-int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx) {
+int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx, int ghost) {
   // Down to 8 planes, the single-simulation minimum: a batch of one or a few
   // small simulations needs short chunks to put a tile on every workgroup.
   static const int cands[] = {128, 96, 64, 48, 32, 24, 16, 12, 8};
@@ -587,7 +589,7 @@ int batch_chunk_x(const int* xs, const int* nty, int n, int slots, int max_lx) {
   std::vector<int> lxs;
   for (int lx : cands) {
     if (lx > max_lx) continue;
-    cost.push_back(batch_makespan(xs, nty, n, slots, lx));
+    cost.push_back(batch_makespan(xs, nty, n, slots, lx, ghost));
     lxs.push_back(lx);
     // Eight or more rounds of tiles already keep every workgroup busy to
     // within an eighth of a round: shorter chunks only add ghost planes.
@@ -633,15 +635,17 @@ size_t batch2_meta_bytes(const BatchItem* items, int n) {
   for (int i = 0; i < n; ++i) npairs = std::max(npairs, (steps_of(*items[i].p) + 1) / 2);
   return align256(sizeof(Stream2Args) * 2 * (size_t)n) + align256(sizeof(Stream2Rare) * (size_t)n) +
          align256(sizeof(int) * (size_t)(n + 2) * 2) +
-         align256(sizeof(int) * 2 * (size_t)std::max(npairs, 1)) + align256(sizeof(int)) +
-         align256(sizeof(int) * 2 * (size_t)kMaxChunkPairs);
+         align256(sizeof(int) * 2 * kQueues * (size_t)std::max(npairs, 1)) + align256(sizeof(int)) +
+         align256(sizeof(int) * 2 * kQueues * (size_t)kMaxChunkPairs);
 }
 
 hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan& plan,
-                          hipStream_t s) {
+                          hipStream_t s, bool forward_only) {
   std::vector<int> cls[2];
   plan.n = n;
   plan.npairs = 0;
+  plan.depth0 = 2;
+  plan.nlaunch0 = 0;
   plan.meta = meta;
   plan.cus = device_cus();
   plan.graph = nullptr;
@@ -654,8 +658,8 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
   }
   // Host image of the argument views, launch data and tile prefix sums,
   // copied once.  Layout: args (2 per sim, class order) | rare (class order)
-  // | tile starts (n_c + 1 per class) | queue counters (2 per pair) | graph
-  // chunk step base | graph chunk queue counters.
+  // | tile starts (n_c + 1 per class) | queue counters (2 x kQueues per
+  // pair) | graph chunk step base | graph chunk queue counters.
   plan.args_bytes = align256(sizeof(Stream2Args) * 2 * (size_t)n);
   plan.rare_bytes = align256(sizeof(Stream2Rare) * (size_t)n);
   plan.ts_bytes = align256(sizeof(int) * (size_t)(n + 2) * 2);
@@ -675,12 +679,12 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
     // one alone runs 11% faster with 8-row tiles, two or more 3-26% slower.
     std::vector<int> xs;
     for (int i : cls[c]) xs.push_back(items[i].g.X);
-    auto chunk = [&](int wy, int resident) {
+    auto chunk = [&](int wy, int resident, int ghost) {
       std::vector<int> ntys;
       for (int i : cls[c]) ntys.push_back((items[i].g.Y + wy - 1) / wy);
       return xs.empty() ? kBatchLX
                         : batch_chunk_x(xs.data(), ntys.data(), (int)xs.size(),
-                                        plan.cus * resident, kBatchLX);
+                                        plan.cus * resident, kBatchLX, ghost);
     };
     int WY = c == 0 ? kBatchWY1 : kBatchWY2;
     if (c == 0) {
@@ -700,10 +704,29 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
       const bool small = forced > 0 ? forced == kBatchWY1Small
                                     : tiles13 < (long long)plan.cus * batch2_resident<1, kBatchWY1>();
       if (small) WY = kBatchWY1Small;
+      // Forward-only batches without graph chunks take three steps per
+      // launch at one cell per lane when the class's 11-row tiles fill every
+      // workgroup at 8-plane chunks (as single calls do on grids that fill
+      // the device).
+      if (forward_only && plan.chunk == 0 && forced <= 0 && !small) {
+        const int wy3 = deep_batch_rows();
+        long long tiles = 0;
+        int maxsteps = 0;
+        for (int i : cls[c]) {
+          tiles += (long long)((items[i].g.Y + wy3 - 1) / wy3) * ((items[i].g.X + 7) / 8);
+          maxsteps = std::max(maxsteps, steps_of(*items[i].p));
+        }
+        if (tiles >= (long long)plan.cus * deep_batch_resident()) {
+          WY = wy3;
+          plan.depth0 = 3;
+          plan.nlaunch0 = (maxsteps + 2) / 3;
+        }
+      }
     }
-    const int cLX = c == 1                       ? chunk(kBatchWY2, batch2_resident<2, kBatchWY2>())
-                    : WY == kBatchWY1Small ? chunk(WY, batch2_resident<1, kBatchWY1Small>())
-                                           : chunk(WY, batch2_resident<1, kBatchWY1>());
+    const int cLX = c == 1            ? chunk(kBatchWY2, batch2_resident<2, kBatchWY2>(), 2)
+                    : plan.depth0 == 3 ? chunk(WY, deep_batch_resident(), 3)
+                    : WY == kBatchWY1Small ? chunk(WY, batch2_resident<1, kBatchWY1Small>(), 2)
+                                           : chunk(WY, batch2_resident<1, kBatchWY1>(), 2);
     plan.wy[c] = WY;
     int t = 0;
     for (size_t k = 0; k < cls[c].size(); ++k) {
@@ -728,31 +751,31 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
   return hipStreamSynchronize(s);
 }
 
-// The launches of one step pair (one per class), queue counters ctr[0..1]
-// (zero on entry), first step *step_base + step_off (step_base may be null).
+// The launches of one step pair (one per class), queue counters
+// ctr[c * kQueues ..] of class c (zero on entry), first step *step_base + step_off (step_base may be null).
 static void launch_pair(const Batch2Plan& plan, hipStream_t s, int* ctr, const int* step_base,
-                        int step_off) {
+                        int step_off, int classes = 3) {
   const Stream2Args* dargs = (const Stream2Args*)plan.meta;
   const int* dts = (const int*)(plan.meta + plan.args_bytes + plan.rare_bytes);
   for (int c = 0; c < 2; ++c) {
     const int nc = plan.count[c];
-    if (nc == 0) continue;
+    if (nc == 0 || !(classes & (1 << c))) continue;
     const Stream2Args* a = dargs + 2 * plan.base[c];
     const int* ts = dts + plan.base[c] + c;
     if (c == 0 && plan.wy[0] == kBatchWY1Small) {
       const int grid = std::min(plan.total[c], plan.cus * batch2_resident<1, kBatchWY1Small>());
       hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY1Small, true>), dim3(grid),
                          dim3(64 * (kBatchWY1Small + 3)), 0, s, a, ts, nc, plan.total[c],
-                         ctr + c, step_base, step_off);
+                         ctr + c * kQueues, step_base, step_off);
     } else if (c == 0) {
       const int grid = std::min(plan.total[c], plan.cus * batch2_resident<1, kBatchWY1>());
       hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY1, true>), dim3(grid),
-                         dim3(64 * (kBatchWY1 + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,
+                         dim3(64 * (kBatchWY1 + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c * kQueues,
                          step_base, step_off);
     } else {
       const int grid = std::min(plan.total[c], plan.cus * batch2_resident<2, kBatchWY2>());
       hipLaunchKernelGGL((k_stream2_batch<2, kBatchWY2, true>), dim3(grid),
-                         dim3(64 * (kBatchWY2 + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c,
+                         dim3(64 * (kBatchWY2 + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c * kQueues,
                          step_base, step_off);
     }
   }
@@ -768,7 +791,8 @@ __global__ void k_add_int(int* p, int v) {
 
 static int* chunk_base(const Batch2Plan& plan) {
   int* counters = (int*)(plan.meta + plan.args_bytes + plan.rare_bytes + plan.ts_bytes);
-  return (int*)((char*)counters + align256(sizeof(int) * 2 * (size_t)std::max(plan.npairs, 1)));
+  return (int*)((char*)counters +
+                align256(sizeof(int) * 2 * kQueues * (size_t)std::max(plan.npairs, 1)));
 }
 
 // Captures one chunk of plan.chunk step pairs as a graph: clear the chunk's
@@ -783,8 +807,8 @@ static hipError_t build_chunk_graph(Batch2Plan& plan) {
   if (e != hipSuccess) return e;
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    (void)launch_zero((float*)ctr, 2 * (size_t)plan.chunk, cs);
-    for (int i = 0; i < plan.chunk; ++i) launch_pair(plan, cs, ctr + 2 * i, base, 2 * i);
+    (void)launch_zero((float*)ctr, 2 * kQueues * (size_t)plan.chunk, cs);
+    for (int i = 0; i < plan.chunk; ++i) launch_pair(plan, cs, ctr + 2 * kQueues * i, base, 2 * i);
     hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, cs, base, 2 * plan.chunk);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cs, &g);
@@ -811,9 +835,28 @@ hipError_t batch2_pairs(Batch2Plan& plan, int m0, int m1, hipStream_t s) {
   }
   int* counters = (int*)(plan.meta + plan.args_bytes + plan.rare_bytes + plan.ts_bytes);
   // Queue counters of these pairs start at zero (a fill kernel).
-  e = launch_zero((float*)(counters + 2 * m0), 2 * (size_t)(m1 - m0), s);
+  e = launch_zero((float*)(counters + 2 * kQueues * m0), 2 * kQueues * (size_t)(m1 - m0), s);
   if (e != hipSuccess) return e;
-  for (int m = m0; m < m1; ++m) launch_pair(plan, s, counters + 2 * m, nullptr, 2 * m);
+  for (int m = m0; m < m1; ++m) launch_pair(plan, s, counters + 2 * kQueues * m, nullptr, 2 * m);
+  return hipGetLastError();
+}
+
+hipError_t batch2_forward(Batch2Plan& plan, hipStream_t s) {
+  if (plan.depth0 != 3) return batch2_pairs(plan, 0, plan.npairs, s);
+  // zz <= 64 class: three steps a launch; the other class in pairs.  Launch
+  // t of the first class uses the queue counters of pair slot t, class 0;
+  // pair m of the second those of slot m, class 1 (nlaunch0 <= npairs).
+  int* counters = (int*)(plan.meta + plan.args_bytes + plan.rare_bytes + plan.ts_bytes);
+  hipError_t e = launch_zero((float*)counters, 2 * kQueues * (size_t)plan.npairs, s);
+  if (e != hipSuccess) return e;
+  const Stream2Args* a = (const Stream2Args*)plan.meta + 2 * plan.base[0];
+  const int* ts = (const int*)(plan.meta + plan.args_bytes + plan.rare_bytes) + plan.base[0];
+  const int grid = std::min(plan.total[0], plan.cus * deep_batch_resident());
+  for (int t = 0; t < plan.nlaunch0; ++t)
+    launch_deep_batch(a, ts, plan.count[0], plan.total[0], counters + 2 * kQueues * t, 3 * t,
+                      grid, s);
+  for (int m = 0; m < plan.npairs; ++m)
+    launch_pair(plan, s, counters + 2 * kQueues * m, nullptr, 2 * m, 2);
   return hipGetLastError();
 }
 
@@ -846,8 +889,8 @@ hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s)
 
 hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) {
   Batch2Plan plan;
-  hipError_t e = batch2_prepare(items, n, meta, plan, s);
-  if (e == hipSuccess) e = batch2_pairs(plan, 0, plan.npairs, s);
+  hipError_t e = batch2_prepare(items, n, meta, plan, s, true);
+  if (e == hipSuccess) e = batch2_forward(plan, s);
   const hipError_t r = batch2_release(plan, s);
   return e != hipSuccess ? e : r;
 }

@@ -240,14 +240,10 @@ __device__ __forceinline__ void planeT(const Stream2Args& a, const Out2& O, LdsT
     for (int k = 0; k < 3; ++k) s.sigm[l - 1][k] = l == 1 ? cur.sig[k] : s.sigm[l > 1 ? l - 2 : 0][k];
 }
 
+// One tile (WY rows x LX planes) of one simulation for steps a.n .. a.n+T-1.
 template <int T, int CPL, int WY, bool vec>
-__global__ __launch_bounds__(64 * (WY + 2 * T - 1)) void k_stream_deep(Stream2Args a) {
+__device__ __forceinline__ void deep_tile(const Stream2Args& a, int tile, LdsT<T, CPL, WY>& L) {
   constexpr int NW = WY + 2 * T - 1;
-  __shared__ LdsT<T, CPL, WY> L;
-  // Tiles adjacent in y (which share ghost rows) on the same XCD, as in the
-  // two-step kernel.
-  int tile = blockIdx.x;
-  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
   const int Z = a.Z;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -299,6 +295,52 @@ __global__ __launch_bounds__(64 * (WY + 2 * T - 1)) void k_stream_deep(Stream2Ar
   }
 }
 
+template <int T, int CPL, int WY, bool vec>
+__global__ __launch_bounds__(64 * (WY + 2 * T - 1)) void k_stream_deep(Stream2Args a) {
+  __shared__ LdsT<T, CPL, WY> L;
+  // Tiles adjacent in y (which share ghost rows) on the same XCD, as in the
+  // two-step kernel.
+  int tile = blockIdx.x;
+  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
+  deep_tile<T, CPL, WY, vec>(a, tile, L);
+}
+
+// Batched form: persistent workgroups take (simulation, tile) items from a
+// device counter, as k_stream2_batch does for step pairs.  Launch t advances
+// every simulation from step T t to T t + T and reads the buffer set t & 1
+// (args holds two views per simulation); a simulation past its last step is
+// skipped, and a launch that runs past a simulation's last step writes only
+// state that is never read (snapshots stop at its last step, sources are not
+// read there).  Forward only: no captures.
+template <int T, int CPL, int WY, bool vec>
+__global__ __launch_bounds__(64 * (WY + 2 * T - 1)) void k_stream_deep_batch(
+    const Stream2Args* __restrict__ args, const int* __restrict__ tile_start, int nsims,
+    int total, int* __restrict__ counter, int step) {
+  __shared__ LdsT<T, CPL, WY> L;
+  __shared__ int s_item, s_next;
+  if (threadIdx.x == 0) s_next = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = pop_item(counter, total, s_next);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (t >= total) break;
+    int lo = 0, hi = nsims - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    Stream2Args a = args[2 * lo + ((step / T) & 1)];
+    if (step >= a.nsteps) continue;
+    a.n = step;
+    a.flags = 0;
+    for (int l = 1; l <= T; ++l)
+      if (out_slot(a.rare, step + l - 1) >= 0) a.flags |= snap_flag(l);
+    deep_tile<T, CPL, WY, vec>(a, t - tile_start[lo], L);
+  }
+}
+
 // Depth T at one cell per lane (zz <= 64), within the store resources'
 // 2^31-byte reach.
 bool stream_deep_supported(const Geom& g, int T) {
@@ -321,6 +363,27 @@ bool stream_deep_preferred(const Geom& g) {
 // for 3 at 512x512x64).
 constexpr int kDeepWY3 = 11;  // stream_deep_preferred's tile count assumes 11
 constexpr int kDeepWY4 = 6;
+
+int deep_batch_rows() { return kDeepWY3; }
+
+int deep_batch_resident() {
+  static const int n = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &b, reinterpret_cast<const void*>(&k_stream_deep_batch<3, 1, kDeepWY3, true>),
+            64 * (kDeepWY3 + 5), 0) != hipSuccess || b < 1)
+      b = 1;
+    return b;
+  }();
+  return n;
+}
+
+void launch_deep_batch(const Stream2Args* args, const int* tile_start, int nsims, int total,
+                       int* counter, int step, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_stream_deep_batch<3, 1, kDeepWY3, true>), dim3(grid),
+                     dim3(64 * (kDeepWY3 + 5)), 0, s, args, tile_start, nsims, total, counter,
+                     step);
+}
 
 template <int T, int CPL, int WY>
 static void launch_deep(const Stream2Args& a, hipStream_t s) {

@@ -332,4 +332,12 @@ Stream2Args make_args2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
                        const Src& src, const OutWin& win, char* ws, const Layout& L, int WY,
                        int LX, int cur);
 
+// Batched three-step launches at one cell per lane (kernels_stream_deep.hip):
+// rows per tile, workgroups resident per CU, and launch t's persistent grid
+// over args (two views per simulation) from step `step`.
+int deep_batch_rows();
+int deep_batch_resident();
+void launch_deep_batch(const Stream2Args* args, const int* tile_start, int nsims, int total,
+                       int* counter, int step, int grid, hipStream_t s);
+
 }  // namespace pjz

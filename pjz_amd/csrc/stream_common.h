@@ -255,4 +255,37 @@ __device__ inline void add_source(const Src& src, const Geom& g, int n, int x, i
   cz += src.axis == 2 ? 0.0f : jv;
 }
 
+// Work queue of the persistent batch kernels.  A launch's items [0, total)
+// (simulation by simulation, a simulation's row tiles of one x chunk
+// consecutive) are split into kQueues contiguous ranges, one per XCD.  A
+// workgroup takes items from the range of the XCD it runs on (the XCC_ID
+// hardware register) first, so y-adjacent tiles, which share ghost rows, run
+// under one L2 cache, and then from the other ranges once its own is empty.
+// (Measured against one shared counter: batch64 1000 steps 6.18 -> 5.83 s,
+// design_batch forward + gradient 2.25 -> 2.15 s; taking blockIdx.x % 8 as
+// the XCD gave 6.02 and 2.16 s.)  `next` (the workgroup's next range to try,
+// 0 = its own) lives in LDS across calls; `counter` holds kQueues zeroed
+// ints.  Called by one lane of the workgroup; returns `total` once every
+// range is empty.
+constexpr int kQueues = 8;
+
+__device__ inline int xcc_id() {
+  int id;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(id));
+  return id;
+}
+
+__device__ inline int pop_item(int* counter, int total, int& next) {
+  const int xcd = xcc_id() & (kQueues - 1);
+  for (; next < kQueues; ++next) {
+    const int q = (xcd + next) & (kQueues - 1);
+    const int start = (int)((long long)q * total / kQueues);
+    const int len = (int)((long long)(q + 1) * total / kQueues) - start;
+    if (len <= 0) continue;
+    const int t = atomicAdd(counter + q, 1);
+    if (t < len) return start + t;
+  }
+  return total;
+}
+
 }  // namespace pjz
