@@ -22,8 +22,8 @@ simulations batched against 64 sequential calls) and --workload
 design_batch (64 simulations of 128x128x64, forward + gradient, batched
 against sequential calls) and --workload design_batch_384 (the same at the
 full 384x384x128, 8000 steps, in memory-bounded waves), --workload mode (the
-mode solver kernel) and --workload density (pjz's density() on its HIP
-kernels, forward + gradient).
+mode solver kernel), --workload density and --workload render (pjz's
+density() and epsilon() on their HIP kernels, forward + gradient).
 """
 
 import argparse
@@ -498,6 +498,57 @@ def bench_density(args):
   print(json.dumps(line), flush=True)
 
 
+def bench_render(args):
+  """pjz's epsilon() on the HIP render kernels: forward + reverse pass of a
+  3-layer stack magnified 2m = 4 times onto 384 x 384 x 128 cells (the
+  inverse-design size), against the torch restatement of the same function
+  on the same GPU (float32)."""
+  import torch
+  from pjz_amd import _epsilon as E
+  dev = torch.device("cuda", 0)
+  xx = yy = 384
+  zz, m = 128, 2
+  gen = torch.Generator(device=dev).manual_seed(0)
+  layers = 1 + 11 * torch.rand((3, 2 * m * xx, 2 * m * yy), generator=gen, device=dev)
+  pos = torch.tensor([40.3, 55.7], device=dev)
+  g = torch.randn((3, xx, yy, zz), generator=gen, device=dev)
+  k = torch.arange(zz, dtype=torch.float32, device=dev)[:, None]
+  gs = k + torch.tensor([[-0.5, 0.0]], device=dev)
+  ge = k + torch.tensor([[0.5, 1.0]], device=dev)
+
+  def hip():
+    lv, pv = layers.clone().requires_grad_(True), pos.clone().requires_grad_(True)
+    (E.render(lv, pv, gs, ge, m) * g).sum().backward()
+
+  def restated():
+    lv, pv = layers.clone().requires_grad_(True), pos.clone().requires_grad_(True)
+    out = torch.stack([E._render_single(lv, pv, gs, ge, m, a, False) for a in "xyz"])
+    (out * g).sum().backward()
+
+  times = {}
+  for name, fn in (("hip", hip), ("torch", restated)):
+    for _ in range(max(args.warmup, 1)):
+      fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+      fn()
+    torch.cuda.synchronize()
+    times[name] = (time.perf_counter() - t0) / args.steps
+  cells = 3 * xx * yy * zz
+  line = {"metric": "epsilon() render forward + gradient, Yee-cell components per second",
+          "unit": "Mcomponent/s", "value": round(cells / times["hip"] / 1e6, 1), "n_gpus": 1,
+          "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(times["hip"] * 1e3, 3),
+          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+          "data": "synthetic",
+          "config": {"workload": f"render: 3 layers of {2 * m * xx}x{2 * m * yy} sub-pixels "
+                                 f"(2m = {2 * m}) -> (3, {xx}, {yy}, {zz}), forward + reverse",
+                     "hip_ms": round(times["hip"] * 1e3, 3),
+                     "torch_restated_ms": round(times["torch"] * 1e3, 3),
+                     "speedup_vs_torch": round(times["torch"] / times["hip"], 2)}}
+  print(json.dumps(line), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -526,6 +577,8 @@ def main():
     return bench_mode(args)
   if args.workload == "density":
     return bench_density(args)
+  if args.workload == "render":
+    return bench_render(args)
 
   import numpy as np
   import torch

@@ -242,6 +242,36 @@ PJZ_FDTD_API int pjz_fdtd_density_backward(int32_t nb, int32_t xx, int32_t yy, f
                                            void* stream);
 
 /*
+ * pjz's permittivity render (src/pjz/_epsilon.py:10-108; epsilon() is the
+ * case grid_start = k + (-1/2, 0), grid_end = k + (1/2, 1), :152-155): ll <=
+ * PJZ_FDTD_RENDER_MAX_LAYERS layered profiles `layers` (ll, 2m xx, 2m yy),
+ * ll - 1 increasing interface `positions`, z cell bounds `grid_start`,
+ * `grid_end` (zz, 2; column 0 for the x and y components, column 1 for z)
+ * -> `eps` (3, xx, yy, zz) by the diagonal sub-pixel average, or the plain
+ * volume average when `simple` is non-zero.  fp32 device arrays, last index
+ * fastest.  The forward also writes `stats` (12 ll xx yy floats), which the
+ * backward takes back with dL/deps to write dL/dlayers and dL/dpositions
+ * (the vector-Jacobian product jax.grad forms through the render; none for
+ * the z grids), with a workspace of pjz_fdtd_render_workspace_bytes
+ * (256-byte aligned).
+ */
+#define PJZ_FDTD_RENDER_MAX_LAYERS 16
+PJZ_FDTD_API int pjz_fdtd_render_workspace_bytes(int32_t ll, int32_t xx, int32_t yy, int32_t zz,
+                                                 size_t* bytes);
+PJZ_FDTD_API int pjz_fdtd_render_forward(int32_t ll, int32_t m, int32_t xx, int32_t yy,
+                                         int32_t zz, int32_t simple, const float* layers,
+                                         const float* positions, const float* grid_start,
+                                         const float* grid_end, float* eps, float* stats,
+                                         void* stream);
+PJZ_FDTD_API int pjz_fdtd_render_backward(int32_t ll, int32_t m, int32_t xx, int32_t yy,
+                                          int32_t zz, int32_t simple, const float* layers,
+                                          const float* positions, const float* grid_start,
+                                          const float* grid_end, const float* stats,
+                                          const float* g_eps, float* g_layers,
+                                          float* g_positions, void* workspace,
+                                          size_t workspace_bytes, void* stream);
+
+/*
  * Diagnostic: the stepper (PJZ_FDTD_KERNEL_REFERENCE .. STREAM4) a forward
  * call with this problem runs on the current device: AUTO and AUTOTUNE
  * resolved (AUTOTUNE to its cached choice, or AUTO before the first call).

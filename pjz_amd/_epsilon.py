@@ -12,8 +12,11 @@ share of a cell comes from clipping its interface interval to the cell
 average: ``1 / (p aoi + (1 - p) / avg)`` with ``p`` the squared share of the
 component's axis in the gradient (``:72-99``), or the plain volume average.
 
-Everything is torch, so the render runs where ``layers`` lives and is
-differentiable with respect to the layers and the interface positions.
+A float32 stack on the GPU runs the HIP kernels of
+``csrc/kernels_render.hip`` (forward and exact reverse pass for the layers
+and the interface positions); a CPU stack runs the torch restatement below
+on the host.  Both are differentiable with respect to the layers and the
+interface positions.
 """
 
 import torch
@@ -74,6 +77,54 @@ def _render_single(layers, layer_pos, grid_start, grid_end, m, axis, use_simple_
   return 1 / (pii * aoi + (1 - pii) * ioa)
 
 
+class _RenderKernel(torch.autograd.Function):
+  """render() through the C ABI (``pjz_fdtd_render_forward`` / ``_backward``)
+  for a float32 stack on the GPU."""
+
+  @staticmethod
+  def forward(ctx, layers, pos, gs, ge, m, simple):
+    from . import _lib
+    layers, pos = layers.detach().contiguous(), pos.detach().contiguous()
+    gs, ge = gs.detach().contiguous(), ge.detach().contiguous()
+    ll, sx, sy = layers.shape
+    xx, yy, zz = sx // (2 * m), sy // (2 * m), gs.shape[0]
+    eps = torch.empty((3, xx, yy, zz), dtype=layers.dtype, device=layers.device)
+    stats = torch.empty(12 * ll * xx * yy, dtype=layers.dtype, device=layers.device)
+    with torch.cuda.device(layers.device):
+      stream = torch.cuda.current_stream(layers.device).cuda_stream
+      _lib.check(_lib.load().pjz_fdtd_render_forward(
+          ll, m, xx, yy, zz, int(simple), layers.data_ptr(), pos.data_ptr() if ll > 1 else None,
+          gs.data_ptr(), ge.data_ptr(), eps.data_ptr(), stats.data_ptr(), stream),
+          "pjz_fdtd_render_forward")
+    ctx.save_for_backward(layers, pos, gs, ge, stats)
+    ctx.dims = (ll, m, xx, yy, zz, int(simple))
+    return eps
+
+  @staticmethod
+  def backward(ctx, g_eps):
+    from . import _lib
+    import ctypes
+    layers, pos, gs, ge, stats = ctx.saved_tensors
+    ll, m, xx, yy, zz, simple = ctx.dims
+    lib = _lib.load()
+    nbytes = ctypes.c_size_t(0)
+    _lib.check(lib.pjz_fdtd_render_workspace_bytes(ll, xx, yy, zz, ctypes.byref(nbytes)),
+               "pjz_fdtd_render_workspace_bytes")
+    # torch's caching allocator hands out 512-byte aligned blocks.
+    ws = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=layers.device)
+    g_eps = g_eps.to(layers.dtype).contiguous()
+    g_layers = torch.empty_like(layers)
+    g_pos = torch.zeros_like(pos)
+    with torch.cuda.device(layers.device):
+      stream = torch.cuda.current_stream(layers.device).cuda_stream
+      _lib.check(lib.pjz_fdtd_render_backward(
+          ll, m, xx, yy, zz, simple, layers.data_ptr(), pos.data_ptr() if ll > 1 else None,
+          gs.data_ptr(), ge.data_ptr(), stats.data_ptr(), g_eps.data_ptr(), g_layers.data_ptr(),
+          g_pos.data_ptr() if ll > 1 else None, ws.data_ptr(), nbytes.value, stream),
+          "pjz_fdtd_render_backward")
+    return g_layers, g_pos, None, None, None, None
+
+
 def render(layers, interface_positions, grid_start, grid_end, m, use_simple_averaging=False):
   """(3, xx, yy, zz) permittivity for explicit z grids (``_epsilon.py:102-108``).
 
@@ -91,6 +142,13 @@ def render(layers, interface_positions, grid_start, grid_end, m, use_simple_aver
   if pos.shape[0] != layers.shape[0] - 1:
     raise ValueError(f"{layers.shape[0]} layers need {layers.shape[0] - 1} interface positions, "
                      f"got {pos.shape[0]}")
+  if layers.is_cuda:
+    if layers.dtype != torch.float32:
+      raise ValueError(f"render() on the GPU computes in float32, got {layers.dtype}")
+    if gs.requires_grad or ge.requires_grad:
+      raise ValueError("render() on the GPU differentiates the layers and interface positions, "
+                       "not the z grids")
+    return _RenderKernel.apply(layers, pos, gs, ge, int(m), bool(use_simple_averaging))
   return torch.stack([_render_single(layers, pos, gs, ge, m, a, use_simple_averaging)
                       for a in "xyz"])
 

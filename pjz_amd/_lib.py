@@ -33,6 +33,9 @@ EXPORTS = (
     "pjz_fdtd_density_workspace_bytes",
     "pjz_fdtd_density_forward",
     "pjz_fdtd_density_backward",
+    "pjz_fdtd_render_workspace_bytes",
+    "pjz_fdtd_render_forward",
+    "pjz_fdtd_render_backward",
     "pjz_fdtd_mode_workspace_bytes",
     "pjz_fdtd_mode_solve",
 )
@@ -113,6 +116,12 @@ def load():
     lib.pjz_fdtd_density_forward.argtypes = [i32] * 3 + [f32] * 6 + [vp] * 4 + [vp]
     lib.pjz_fdtd_density_backward.restype = ctypes.c_int
     lib.pjz_fdtd_density_backward.argtypes = [i32] * 3 + [f32] * 6 + [vp] * 4 + [vp, sz, vp]
+    lib.pjz_fdtd_render_workspace_bytes.restype = ctypes.c_int
+    lib.pjz_fdtd_render_workspace_bytes.argtypes = [i32] * 4 + [ctypes.POINTER(sz)]
+    lib.pjz_fdtd_render_forward.restype = ctypes.c_int
+    lib.pjz_fdtd_render_forward.argtypes = [i32] * 6 + [vp] * 6 + [vp]
+    lib.pjz_fdtd_render_backward.restype = ctypes.c_int
+    lib.pjz_fdtd_render_backward.argtypes = [i32] * 6 + [vp] * 8 + [vp, sz, vp]
     if lib.pjz_fdtd_abi_version() != ABI_VERSION:
       raise RuntimeError("libpjzfdtd.so ABI version mismatch; rebuild it")
     _lib = lib

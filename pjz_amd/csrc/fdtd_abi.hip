@@ -444,6 +444,65 @@ int pjz_fdtd_density_backward(int32_t nb, int32_t xx, int32_t yy, float radius, 
   return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "density backward");
 }
 
+namespace {
+int check_render(int32_t ll, int32_t m, int32_t xx, int32_t yy, int32_t zz) {
+  if (ll < 1 || ll > PJZ_FDTD_RENDER_MAX_LAYERS)
+    return fail(PJZ_FDTD_EINVAL, "render: 1 <= layers <= %d, got %d", PJZ_FDTD_RENDER_MAX_LAYERS,
+                ll);
+  if (m < 1 || xx < 1 || yy < 1 || zz < 1)
+    return fail(PJZ_FDTD_EINVAL, "render: m, xx, yy, zz must be >= 1, got %d, %d, %d, %d", m, xx,
+                yy, zz);
+  if ((long long)ll * 4 * m * m * xx * yy >= (1LL << 31) / 4 ||
+      3LL * xx * yy * zz >= (1LL << 31) / 4)
+    return fail(PJZ_FDTD_EINVAL, "render: arrays too large");
+  return PJZ_FDTD_OK;
+}
+}  // namespace
+
+int pjz_fdtd_render_workspace_bytes(int32_t ll, int32_t xx, int32_t yy, int32_t zz,
+                                    size_t* bytes) {
+  int rc = check_render(ll, 1, xx, yy, zz);
+  if (rc) return rc;
+  if (!bytes) return fail(PJZ_FDTD_EINVAL, "bytes is NULL");
+  *bytes = pjz::render_workspace_bytes(ll, xx, yy, zz);
+  return PJZ_FDTD_OK;
+}
+
+int pjz_fdtd_render_forward(int32_t ll, int32_t m, int32_t xx, int32_t yy, int32_t zz,
+                            int32_t simple, const float* layers, const float* positions,
+                            const float* grid_start, const float* grid_end, float* eps,
+                            float* stats, void* stream) {
+  int rc = check_render(ll, m, xx, yy, zz);
+  if (rc) return rc;
+  if (!layers || (ll > 1 && !positions) || !grid_start || !grid_end || !eps || !stats)
+    return fail(PJZ_FDTD_EINVAL, "render: a required pointer argument is NULL");
+  hipError_t e = pjz::render_forward(ll, m, xx, yy, zz, simple != 0, layers, positions,
+                                     grid_start, grid_end, eps, stats, (hipStream_t)stream);
+  return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "render forward");
+}
+
+int pjz_fdtd_render_backward(int32_t ll, int32_t m, int32_t xx, int32_t yy, int32_t zz,
+                             int32_t simple, const float* layers, const float* positions,
+                             const float* grid_start, const float* grid_end, const float* stats,
+                             const float* g_eps, float* g_layers, float* g_positions,
+                             void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_render(ll, m, xx, yy, zz);
+  if (rc) return rc;
+  if (!layers || (ll > 1 && (!positions || !g_positions)) || !grid_start || !grid_end ||
+      !stats || !g_eps || !g_layers || !workspace)
+    return fail(PJZ_FDTD_EINVAL, "render: a required pointer argument is NULL");
+  const size_t need = pjz::render_workspace_bytes(ll, xx, yy, zz);
+  if (workspace_bytes < need)
+    return fail(PJZ_FDTD_ESPACE, "render: workspace of %zu bytes, %zu needed", workspace_bytes,
+                need);
+  if (((uintptr_t)workspace & 255) != 0)
+    return fail(PJZ_FDTD_EINVAL, "render: workspace must be 256-byte aligned");
+  hipError_t e = pjz::render_backward(ll, m, xx, yy, zz, simple != 0, layers, positions,
+                                      grid_start, grid_end, stats, g_eps, g_layers, g_positions,
+                                      (char*)workspace, (hipStream_t)stream);
+  return e == hipSuccess ? PJZ_FDTD_OK : hip_fail(e, "render backward");
+}
+
 int pjz_fdtd_stepper(const pjz_fdtd_problem* p, int32_t* kernel) {
   int rc = check(p);
   if (rc) return rc;

@@ -285,6 +285,21 @@ hipError_t density_backward(int nb, int X, int Y, float radius, float alpha, flo
                             float lo, float hi, const float* uf, const float* gd,
                             const float* gl, float* gu, char* ws, hipStream_t s);
 
+// pjz's render (kernels_render.hip): ll layers (ll, 2m xx, 2m yy), ll - 1
+// interface positions, z grids (zz, 2); eps (3, xx, yy, zz); stats of
+// render_stats_floats written by the forward and read by the backward, which
+// needs a workspace of render_workspace_bytes.
+int render_max_layers();
+size_t render_stats_floats(int ll, int xx, int yy);
+size_t render_workspace_bytes(int ll, int xx, int yy, int zz);
+hipError_t render_forward(int ll, int m, int xx, int yy, int zz, int simple, const float* layers,
+                          const float* pos, const float* gs, const float* ge, float* eps,
+                          float* stats, hipStream_t s);
+hipError_t render_backward(int ll, int m, int xx, int yy, int zz, int simple, const float* layers,
+                           const float* pos, const float* gs, const float* ge,
+                           const float* stats, const float* g_eps, float* g_layers, float* g_pos,
+                           char* ws, hipStream_t s);
+
 // lE[window] += g (one snapshot cotangent, (3, ex, ey, ez)).
 hipError_t launch_inject(const Geom& g, const pjz_fdtd_problem& p, float* lE, const float* g_out,
                          hipStream_t s);

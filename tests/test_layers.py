@@ -136,3 +136,59 @@ def test_render_on_the_gpu(gpu):
                    torch.tensor([thresh[2]], device=gpu),
                    *(g.to(gpu) for g in zgrid(2, torch.float32)), m=1)
     assert float(out[index]) == pytest.approx(expected, rel=1e-5)
+
+
+@pytest.mark.gpu
+def test_render_kernel_reference_cases(gpu):
+  """Every literal case of the reference's tests through the HIP render
+  (float32 on the GPU): the simple, x-threshold, z-threshold and corner
+  families."""
+  f32 = torch.float32
+  zg = lambda zz: tuple(g.to(gpu) for g in zgrid(zz, f32))
+  for thresh, index, expected in SIMPLE:
+    out = E.render(two_layers(4, 5, 1, thresh, dtype=f32).to(gpu),
+                   torch.tensor([float(thresh[2])], device=gpu), *zg(6), 1)
+    assert float(out[index]) == pytest.approx(expected, rel=1e-5), (thresh, index)
+  for m, thresh, component, index, expected in X_THRESHOLD:
+    layer = torch.ones((1, 2 * m * 4, 2 * m * 3), dtype=f32)
+    layer[0, thresh:, :] = 3
+    out = E.render(layer.to(gpu), torch.zeros(0, device=gpu), *zg(2), m)
+    assert float(out[component, index, 0, 0]) == pytest.approx(expected, rel=1e-5)
+  for thresh, component, index, expected in Z_THRESHOLD:
+    layer = torch.ones((2, 2, 2), dtype=f32)
+    layer[1] = 3
+    out = E.render(layer.to(gpu), torch.tensor([float(thresh)], device=gpu), *zg(1), m=1)
+    assert float(out[component, 0, 0, index]) == pytest.approx(expected, rel=1e-5)
+  for thresh, index, expected in CORNER:
+    out = E.render(two_layers(2, 2, 1, thresh, dtype=f32).to(gpu),
+                   torch.tensor([float(thresh[2])], device=gpu), *zg(2), m=1)
+    assert float(out[index]) == pytest.approx(expected, rel=1e-5), (thresh, index)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pos,m,simple", [([1.3, 2.6], 2, False), ([2.0, 3.5], 1, False),
+                                          ([1.3, 2.6], 2, True), ([], 3, False)])
+def test_render_kernel_gradient(gpu, pos, m, simple):
+  """The HIP reverse pass against torch.autograd through the float64 torch
+  restatement on the CPU, for a random cotangent: interfaces inside cells,
+  interfaces on cell bounds of both z grids (the clip's half gradients at
+  ties), the simple average, a single layer."""
+  rng = np.random.default_rng(len(pos) + m)
+  ll = len(pos) + 1
+  xx, yy, zz = 5, 4, 6
+  layers = 1 + 11 * rng.random((ll, 2 * m * xx, 2 * m * yy))
+  g = rng.standard_normal((3, xx, yy, zz))
+  lc = torch.tensor(layers, requires_grad=True)
+  pc = torch.tensor(pos, dtype=torch.float64, requires_grad=True)
+  out = E.render(lc, pc, *zgrid(zz), m, use_simple_averaging=simple)
+  (out * torch.tensor(g)).sum().backward()
+  lg = torch.tensor(layers, dtype=torch.float32, device=gpu, requires_grad=True)
+  pg = torch.tensor(pos, dtype=torch.float32, device=gpu, requires_grad=True)
+  og = E.render(lg, pg, *(t.to(gpu) for t in zgrid(zz, torch.float32)), m,
+                use_simple_averaging=simple)
+  np.testing.assert_allclose(og.detach().cpu().numpy(), out.detach().numpy(), rtol=1e-5)
+  (og * torch.tensor(g, dtype=torch.float32, device=gpu)).sum().backward()
+  got, ref = lg.grad.cpu().numpy(), lc.grad.numpy()
+  assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+  if pos:
+    np.testing.assert_allclose(pg.grad.cpu().numpy(), pc.grad.numpy(), rtol=1e-4, atol=1e-5)
