@@ -83,7 +83,13 @@ typedef struct pjz_fdtd_problem {
                          /* steps so pjz_fdtd_backward can run on the workspace  */
   int32_t graph_steps;   /* batches: steps per captured, replayed hipGraph    */
                          /* chunk (even, <= 128); 0 = one launch per pair   */
-  int32_t reserved;
+  int32_t ckpt_inner;    /* 0 < ckpt_inner < ckpt_every: two-level checkpoints  */
+                         /* (single calls): the forward keeps every ckpt_every-th */
+                         /* state; the backward recomputes each such segment     */
+                         /* keeping every ckpt_inner-th, then recomputes those   */
+                         /* sub-segments capturing E changes (one more forward,  */
+                         /* workspace ~ (steps/every + every/inner) states +     */
+                         /* inner E changes instead of steps/every + every)     */
 } pjz_fdtd_problem;
 
 /* ABI version of the loaded library (PJZ_FDTD_ABI_VERSION). */

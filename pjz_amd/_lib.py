@@ -56,7 +56,7 @@ class Problem(ctypes.Structure):
       ("flags", ctypes.c_int32), ("kernel", ctypes.c_int32),
       ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32),
       ("waves_per_simd", ctypes.c_int32), ("ckpt_every", ctypes.c_int32),
-      ("graph_steps", ctypes.c_int32), ("reserved", ctypes.c_int32),
+      ("graph_steps", ctypes.c_int32), ("ckpt_inner", ctypes.c_int32),
   ]
 
 

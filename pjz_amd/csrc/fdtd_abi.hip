@@ -91,6 +91,11 @@ int check(const pjz_fdtd_problem* p) {
     return fail(PJZ_FDTD_EINVAL, "the three-step stream kernel needs zz <= 64, got %d", p->zz);
   if (p->kernel == PJZ_FDTD_KERNEL_STREAM4 && !pjz::stream_deep_supported(geom_of(*p), 4))
     return fail(PJZ_FDTD_EINVAL, "the four-step stream kernel needs zz <= 64, got %d", p->zz);
+  if (p->ckpt_every < 0 || p->ckpt_inner < 0)
+    return fail(PJZ_FDTD_EINVAL, "ckpt_every and ckpt_inner must be >= 0, got %d, %d",
+                p->ckpt_every, p->ckpt_inner);
+  if (p->ckpt_inner > 0 && p->ckpt_every <= 0)
+    return fail(PJZ_FDTD_EINVAL, "ckpt_inner %d needs ckpt_every > 0", p->ckpt_inner);
   return PJZ_FDTD_OK;
 }
 
@@ -571,7 +576,7 @@ int pjz_fdtd_forward(const pjz_fdtd_problem* p, const float* epsilon,
   if (L.nck > 0) {
     // Checkpoint the state before steps 0, S, 2S, ... for pjz_fdtd_backward.
     for (int k = 0; k < L.nck && e == hipSuccess; ++k) {
-      const int b = k * L.S, end = b + L.S < nsteps ? b + L.S : nsteps;
+      const int b = k * L.S1, end = b + L.S1 < nsteps ? b + L.S1 : nsteps;
       e = save_state(g, L, ws, cur, k, s);
       if (e == hipSuccess) e = run_steps(g, *p, absorption_mask, src, win, ws, L, b, end, &cur,
                                          nullptr, s);
@@ -651,6 +656,9 @@ int batch_plan(const pjz_fdtd_problem* probs, int n, BatchPlanHost& bp) {
                   bp.S);
     if (S % 2 != 0)
       return fail(PJZ_FDTD_EINVAL, "batch item %d: ckpt_every must be even, got %d", i, S);
+    if (probs[i].ckpt_inner != 0)
+      return fail(PJZ_FDTD_EINVAL, "batch item %d: batches keep one level of checkpoints "
+                  "(ckpt_inner must be 0, got %d)", i, probs[i].ckpt_inner);
     pjz::Geom g = geom_of(probs[i]);
     if (!pjz::batch2_supported(g))
       return fail(PJZ_FDTD_EINVAL, "batch item %d: zz must be <= 64 or even and <= 128, got %d", i,
@@ -923,16 +931,16 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
   const long long gstride = 3LL * p->ex * p->ey * p->ez;
   int acur = 0;  // adjoint set holding the current cotangents
   (void)P;
-  for (int k = L.nck - 1; k >= 0 && e == hipSuccess; --k) {
-    const int b = k * L.S, end = b + L.S < nsteps ? b + L.S : nsteps;
-    // Recompute the segment from its checkpoint, capturing D^n = E^n - E^(n+1).
+  auto is_out = [&](int n) {
+    return n >= p->out_start && n < p->out_stop && (n - p->out_start) % p->out_step == 0;
+  };
+  // Steps [b, end) from the state in checkpoint slot k: recompute them
+  // capturing D^n = E^n - E^(n+1), then reverse them.
+  auto reverse_segment = [&](int k, int b, int end) {
     int cur = 0;
-    e = restore_state(g, L, ws, k, s);
+    hipError_t e = restore_state(g, L, ws, k, s);
     if (e == hipSuccess)
       e = run_steps(g, *p, absorption_mask, src, nowin, ws, L, b, end, &cur, seg, s);
-    auto is_out = [&](int n) {
-      return n >= p->out_start && n < p->out_stop && (n - p->out_start) % p->out_step == 0;
-    };
     for (int n = end - 1; n >= b && e == hipSuccess;) {
       if (is_out(n))
         e = pjz::launch_inject(g, *p, (float*)(ws + L.lE_[acur]),
@@ -953,6 +961,30 @@ int pjz_fdtd_backward(const pjz_fdtd_problem* p, const float* epsilon,
         e = pjz::launch_adjoint_step(g, *p, absorption_mask, ws, L, seg + (n - b) * f3, acur, s);
         --n;
       }
+    }
+    return e;
+  };
+  for (int k = L.nck - 1; k >= 0 && e == hipSuccess; --k) {
+    const int b1 = k * L.S1, e1 = b1 + L.S1 < nsteps ? b1 + L.S1 : nsteps;
+    if (L.nfine == 0) {
+      e = reverse_segment(k, b1, e1);
+      continue;
+    }
+    // Two levels: recompute the S1 segment from its checkpoint keeping the
+    // state every S steps (slots nck + j), then reverse its sub-segments.
+    int cur = 0;
+    e = restore_state(g, L, ws, k, s);
+    int nsub = 0;
+    for (int b = b1; b < e1 && e == hipSuccess; b += L.S, ++nsub) {
+      e = save_state(g, L, ws, cur, L.nck + nsub, s);
+      const int end = b + L.S < e1 ? b + L.S : e1;
+      // The last sub-segment's steps are recomputed again below.
+      if (e == hipSuccess && end < e1)
+        e = run_steps(g, *p, absorption_mask, src, nowin, ws, L, b, end, &cur, nullptr, s);
+    }
+    for (int j = nsub - 1; j >= 0 && e == hipSuccess; --j) {
+      const int b = b1 + j * L.S, end = b + L.S < e1 ? b + L.S : e1;
+      e = reverse_segment(L.nck + j, b, end);
     }
   }
   if (e == hipSuccess)

@@ -57,6 +57,11 @@ struct Layout {
   size_t accG;        // gradient accumulator sum_n lE^(n+1) (E^n - E^(n+1)) (3N)
   size_t total;
   int S, nck;
+  // Two-level checkpoints: the forward keeps nck states every S1 steps
+  // (slots 0 .. nck-1); the backward keeps up to nfine states every S steps
+  // of one S1 segment in slots nck .. nck+nfine-1.  One level: S1 = S,
+  // nfine = 0.
+  int S1, nfine;
   size_t ckpt_floats;  // E (3N) + H (3N) + psi_H pair (2P) + psi_E pair (2P)
 };
 
@@ -86,14 +91,22 @@ inline Layout make_layout(const Geom& g, const pjz_fdtd_problem& p) {
   L.rare = off; off += 256;
   L.S = 0;
   L.nck = 0;
+  L.S1 = 0;
+  L.nfine = 0;
   L.ckpt_floats = 6 * (size_t)g.N + 4 * P;
   L.ckpt = L.seg = L.accG = off;
   for (int i = 0; i < 2; ++i) L.lE_[i] = L.lH_[i] = L.lu_[i] = L.lpE_[i] = off;
   const int nsteps = steps_of(p);
   if (p.ckpt_every > 0 && nsteps > 0) {
-    L.S = p.ckpt_every < nsteps ? p.ckpt_every : nsteps;
-    L.nck = (nsteps + L.S - 1) / L.S;
-    L.ckpt = off; off += align256(sizeof(float) * L.ckpt_floats) * (size_t)L.nck;
+    L.S1 = p.ckpt_every < nsteps ? p.ckpt_every : nsteps;
+    L.nck = (nsteps + L.S1 - 1) / L.S1;
+    L.S = L.S1;
+    if (p.ckpt_inner > 0 && p.ckpt_inner < L.S1) {
+      L.S = p.ckpt_inner;
+      L.nfine = (L.S1 + L.S - 1) / L.S;
+    }
+    L.ckpt = off;
+    off += align256(sizeof(float) * L.ckpt_floats) * (size_t)(L.nck + L.nfine);
     // D^b .. D^(b+S-1) of a segment, plus one slot for the extra step a
     // batched pair runs past an odd step count.
     L.seg = off; off += f3 * (size_t)(L.S + 1);

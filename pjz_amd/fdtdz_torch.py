@@ -23,7 +23,8 @@ _KERNELS = {"auto": _lib.KERNEL_AUTO, "reference": _lib.KERNEL_REFERENCE,
             "stream": _lib.KERNEL_STREAM, "stream2": _lib.KERNEL_STREAM2,
             "stream3": _lib.KERNEL_STREAM3, "stream4": _lib.KERNEL_STREAM4,
             "autotune": _lib.KERNEL_AUTOTUNE}
-_LAUNCH_KEYS = {"kernel", "tile_x", "tile_y", "waves_per_simd", "ckpt_every", "graph_steps"}
+_LAUNCH_KEYS = {"kernel", "tile_x", "tile_y", "waves_per_simd", "ckpt_every", "ckpt_inner",
+                "memory_budget", "graph_steps"}
 
 
 def _launch_options(launch_params):
@@ -35,8 +36,13 @@ def _launch_options(launch_params):
   fastest), ``tile_x``, ``tile_y``,
   ``waves_per_simd`` (stream kernel tiling), ``ckpt_every`` (steps between
   checkpoints kept for the gradient; default about sqrt(2 * steps)),
-  ``graph_steps`` (batches: steps per captured and replayed hipGraph chunk,
-  even, <= 128; 0, the default, launches every step pair directly).
+  ``ckpt_inner`` (single calls: steps between the second level of
+  checkpoints the backward keeps inside each ``ckpt_every`` segment; 0 = one
+  level), ``memory_budget`` (single calls with a gradient: bytes the
+  workspace may take; default 90% of the device's free memory.  Without an
+  explicit ``ckpt_every``, checkpoints go to two levels when one does not
+  fit), ``graph_steps`` (batches: steps per captured and replayed hipGraph
+  chunk, even, <= 128; 0, the default, launches every step pair directly).
   """
   if launch_params is None:
     return {}
@@ -115,6 +121,7 @@ def make_problem(epsilon_shape, dt, source_shape, source_position, mask_shape,
   p.tile_y = int(opts.get("tile_y", 0))
   p.waves_per_simd = int(opts.get("waves_per_simd", 0))
   p.ckpt_every = int(opts.get("ckpt_every", 0))
+  p.ckpt_inner = int(opts.get("ckpt_inner", 0))
   p.graph_steps = int(opts.get("graph_steps", 0))
   return p
 
@@ -129,6 +136,41 @@ def default_ckpt_every(nsteps):
   """Checkpoint spacing that balances checkpoints (~24 B/cell each) against
   the per-step E of one segment (12 B/cell each): about sqrt(2 * steps)."""
   return max(1, min(nsteps, int(round(math.sqrt(2.0 * max(nsteps, 1))))))
+
+
+def _with_ckpt(prob, every, inner):
+  q = _lib.Problem()
+  ctypes.pointer(q)[0] = prob
+  q.ckpt_every, q.ckpt_inner = every, inner
+  return q
+
+
+def plan_checkpoints(prob, budget):
+  """(ckpt_every, ckpt_inner) for a single call's gradient within ``budget``
+  workspace bytes: one level every ~sqrt(2 steps) steps if that fits, else
+  the two-level spacing with the smallest workspace (one more forward pass in
+  the backward).  Raises ValueError if no spacing fits."""
+  T = steps_of(prob)
+  one = default_ckpt_every(T)
+  if _lib.workspace_bytes(_with_ckpt(prob, one, 0)) <= budget:
+    return one, 0
+  # Workspace = fixed part + (ceil(T/S1) + ceil(S1/S2)) states + (S2 + 1)
+  # E changes: search the inner spacing near its optimum, the outer one
+  # near sqrt(T S2) for each.
+  best = None
+  for s2 in range(1, max(2, int(3 * T ** (1 / 3.0)) + 2)):
+    c = int(round(math.sqrt(T * s2)))
+    for s1 in {max(s2 + 1, c + d) for d in range(-2 * s2, 2 * s2 + 1, max(1, s2 // 2))}:
+      if s1 >= T:
+        continue
+      w = _lib.workspace_bytes(_with_ckpt(prob, s1, s2))
+      if best is None or w < best[0]:
+        best = (w, s1, s2)
+  if best is None or best[0] > budget:
+    need = best[0] if best else _lib.workspace_bytes(_with_ckpt(prob, one, 0))
+    raise ValueError(f"the gradient workspace needs at least {need} bytes with two levels of "
+                     f"checkpoints, above the memory budget of {int(budget)} bytes")
+  return best[1], best[2]
 
 
 def _device_of(arrays):
@@ -253,10 +295,16 @@ def fdtdz(epsilon, dt, source_field, source_waveform, source_position,
   wants_grad = (torch.is_grad_enabled() and isinstance(epsilon, torch.Tensor)
                 and epsilon.requires_grad)
   if not wants_grad:
-    prob.ckpt_every = 0
+    prob.ckpt_every = prob.ckpt_inner = 0
     return _forward(prob, dev, device)[0]
   if prob.ckpt_every <= 0:
-    prob.ckpt_every = default_ckpt_every(steps_of(prob))
+    opts = _launch_options(launch_params)
+    budget = opts.get("memory_budget")
+    if budget is None:
+      free, _ = torch.cuda.mem_get_info(device)
+      cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+      budget = 0.9 * (free + cached)
+    prob.ckpt_every, prob.ckpt_inner = plan_checkpoints(prob, float(budget))
   return FdtdzFunction.apply(epsilon, prob, dev, device)
 
 
@@ -304,6 +352,8 @@ def _subset(probs, idx, ckpt_every=None):
     ctypes.memmove(ctypes.byref(sub[j]), ctypes.byref(probs[i]), ctypes.sizeof(_lib.Problem))
     if ckpt_every is not None:
       sub[j].ckpt_every = ckpt_every
+      if ckpt_every == 0:
+        sub[j].ckpt_inner = 0
   return sub
 
 
@@ -460,7 +510,7 @@ def fdtdz_batch(calls, memory_budget=None):
       isinstance(e, torch.Tensor) and e.requires_grad for e in epsilons)
   if not wants_grad:
     for p in probs:
-      p.ckpt_every = 0
+      p.ckpt_every = p.ckpt_inner = 0
     return _batch_forward(probs, devs, device)[0]
   S = batch_ckpt_every(probs)
   for p in probs:

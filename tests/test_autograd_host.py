@@ -135,3 +135,39 @@ def test_plan_waves_fits_the_budget():
     assert "exceeds the memory budget" in str(e)
   else:
     raise AssertionError("an item larger than the budget must raise")
+
+
+def test_plan_checkpoints_two_levels_when_one_does_not_fit():
+  """Single calls: one level every ~sqrt(2 steps) steps when it fits the
+  budget; else the two-level spacing with the smallest workspace (the
+  library's size query, no GPU), which fits a budget one level cannot meet;
+  and a clear error below that."""
+  p = F.make_problem((3, 512, 512, 128), 0.5, (2, 1, 512, 128), 40, (3, 512, 512), (128, 2),
+                     (16, 16), (9976, 10000, 3), 10000, True, None, (0, 0, 0))
+  one = F.default_ckpt_every(F.steps_of(p))
+  w_one = F._lib.workspace_bytes(F._with_ckpt(p, one, 0))
+  assert F.plan_checkpoints(p, w_one) == (one, 0)
+  s1, s2 = F.plan_checkpoints(p, w_one - 1)
+  assert 0 < s2 < s1 < F.steps_of(p)
+  w_two = F._lib.workspace_bytes(F._with_ckpt(p, s1, s2))
+  assert w_two < 0.6 * w_one
+  assert F.plan_checkpoints(p, w_two) == (s1, s2)
+  try:
+    F.plan_checkpoints(p, w_two - 1)
+  except ValueError as e:
+    assert "memory budget" in str(e)
+  else:
+    raise AssertionError("a budget below the smallest workspace must raise")
+
+
+def test_ckpt_inner_is_checked():
+  p = F.make_problem((3, 40, 30, 64), 0.5, (2, 1, 30, 64), 4, (3, 40, 30), (64, 2), (4, 4),
+                     (90, 100, 5), 100, False, dict(ckpt_inner=5), (0, 0, 0))
+  try:
+    F._lib.workspace_bytes(p)
+  except ValueError as e:
+    assert "ckpt_inner" in str(e)
+  else:
+    raise AssertionError("ckpt_inner without ckpt_every must raise")
+  p.ckpt_every = 20
+  assert F._lib.workspace_bytes(p) < F._lib.workspace_bytes(F._with_ckpt(p, 20, 0))

@@ -156,3 +156,36 @@ def test_gradient_full_size_directional():
         f"extrapolated {fd:.6e}", flush=True)
   assert abs(fd) > 0
   assert abs(analytic - fd) <= 5e-3 * abs(fd)
+
+
+@pytest.mark.parametrize("every,inner", [(24, 8), (24, 7), (30, 29), (1000, 9)])
+def test_two_level_checkpoints_match_one_level(every, inner):
+  """Two levels of checkpoints (the backward recomputes each outer segment
+  keeping inner states, then each inner sub-segment capturing E changes):
+  the same gradient as one level, and the oracle's."""
+  print(f"case every={every} inner={inner}", flush=True)
+  eps, kw, g = problem(0, seed=9)
+  ref = adjoint.grad_epsilon(eps, g_out=g, **kw)
+  one, _ = gpu_grad(eps, kw, g, ckpt_every=every)
+  two, _ = gpu_grad(eps, kw, g, ckpt_every=every, ckpt_inner=inner)
+  assert rel_l2(two, ref) <= TOL
+  # Reversed steps pair up within a recomputed segment only, so the inner
+  # spacing changes which steps share a launch (and the rounding): 1e-5.
+  assert rel_l2(two, one) <= 1e-5
+
+
+def test_memory_budget_selects_two_levels():
+  """A budget below the one-level workspace makes fdtdz() choose two levels
+  of checkpoints by itself; the gradient still matches the oracle."""
+  eps, kw, g = problem(1, seed=10)
+  prob = F.make_problem(eps.shape, kw["dt"], kw["source_field"].shape, kw["source_position"],
+                        kw["absorption_mask"].shape, kw["pml_kappa"].shape, kw["pml_widths"],
+                        kw["output_steps"], kw["source_waveform"].shape[0],
+                        kw["use_reduced_precision"], None, kw["offset"])
+  one = F.default_ckpt_every(F.steps_of(prob))
+  budget = F._lib.workspace_bytes(F._with_ckpt(prob, one, 0)) - 1
+  every, inner = F.plan_checkpoints(prob, budget)
+  assert inner > 0
+  ref = adjoint.grad_epsilon(eps, g_out=g, **kw)
+  got, _ = gpu_grad(eps, kw, g, memory_budget=budget)
+  assert rel_l2(got, ref) <= TOL
