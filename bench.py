@@ -326,7 +326,13 @@ def bench_design_full(args):
   cells = sum(int(c["epsilon"].shape[1] * c["epsilon"].shape[2] * c["epsilon"].shape[3])
               for c in calls)
 
+  t_start = [time.perf_counter()]
+
   def objective(i, out):
+    # Progress on stderr as each simulation's objective is formed (a wave's
+    # forward is done); long runs would otherwise print nothing for minutes.
+    print(f"  simulation {i} forward done at {time.perf_counter() - t_start[0]:.0f} s",
+          file=sys.stderr, flush=True)
     w, mon_x, mode = monitors[i]
     return workloads.overlap_objective(out, mode, [w], calls[i]["output_steps"],
                                        monitor_x=mon_x)
@@ -359,7 +365,7 @@ def bench_design_full(args):
 
   def timed(fn):
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    t0 = t_start[0] = time.perf_counter()
     for _ in range(args.steps):
       r = fn()
     torch.cuda.synchronize()
