@@ -113,3 +113,32 @@ def test_copy_rejects_bad_arguments_before_any_launch():
   assert "16" in lib.pjz_fdtd_last_error().decode()
   aligned = ctypes.c_void_p(base + (16 - base % 16) % 16)
   assert lib.pjz_fdtd_copy(aligned, aligned, 0, None) == _lib.OK
+
+
+def test_density_and_render_reject_bad_arguments_before_any_launch():
+  """Argument checks of the density and render entry points run on the host
+  (no GPU): sizes, radius, layer count, NULL pointers, workspace size."""
+  lib = _lib.load()
+  sz = ctypes.c_size_t(0)
+  assert lib.pjz_fdtd_density_workspace_bytes(1, 16, 12, 2.5, ctypes.byref(sz)) == _lib.OK
+  assert sz.value > 0
+  assert lib.pjz_fdtd_density_workspace_bytes(0, 16, 12, 2.5, ctypes.byref(sz)) == _lib.EINVAL
+  assert lib.pjz_fdtd_density_workspace_bytes(1, 16, 12, 0.0, ctypes.byref(sz)) == _lib.EINVAL
+  assert lib.pjz_fdtd_density_workspace_bytes(1, 16, 12, 41.0, ctypes.byref(sz)) == _lib.EINVAL
+  assert b"radius" in lib.pjz_fdtd_last_error()
+  args = (1, 16, 12, 2.5, 0.5, 1.0, 0.5, 0.25, 0.75)
+  assert lib.pjz_fdtd_density_forward(*args, None, None, None, None, None) == _lib.EINVAL
+  assert lib.pjz_fdtd_density_forward(1, 16, 12, 2.5, 0.5, 1.0, 0.5, 0.8, 0.2,
+                                      None, None, None, None, None) == _lib.EINVAL
+  assert b"eta_lo" in lib.pjz_fdtd_last_error()
+  buf = ctypes.create_string_buffer(1024)
+  p = ctypes.c_void_p(ctypes.addressof(buf))
+  assert lib.pjz_fdtd_density_backward(*args, p, p, p, p, p, 16, None) == _lib.ESPACE
+
+  assert lib.pjz_fdtd_render_workspace_bytes(3, 8, 8, 16, ctypes.byref(sz)) == _lib.OK
+  assert lib.pjz_fdtd_render_workspace_bytes(17, 8, 8, 16, ctypes.byref(sz)) == _lib.EINVAL
+  assert b"layers" in lib.pjz_fdtd_last_error()
+  assert lib.pjz_fdtd_render_forward(2, 0, 8, 8, 16, 0, p, p, p, p, p, p, None) == _lib.EINVAL
+  assert lib.pjz_fdtd_render_forward(2, 1, 8, 8, 16, 0, p, None, p, p, p, p, None) == _lib.EINVAL
+  assert lib.pjz_fdtd_render_backward(2, 1, 8, 8, 16, 0, p, p, p, p, p, p, p, p, p, 16,
+                                      None) == _lib.ESPACE
