@@ -231,7 +231,6 @@ hipError_t batch2_forward(Batch2Plan& plan, hipStream_t s);
 // Frees the plan's graph (synchronising the stream if one was built).
 hipError_t batch2_release(Batch2Plan& plan, hipStream_t s);
 hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s);
-hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s);
 
 // Reverse of step n: adjoint set `acur` (lE, lH, lu, lpE) holds the
 // cotangents after step n on entry and before it on exit (in place); G

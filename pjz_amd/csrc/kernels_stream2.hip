@@ -887,12 +887,4 @@ hipError_t batch2_capture_from(const Batch2Plan& plan, int begin, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t run_batch2(const BatchItem* items, int n, char* meta, hipStream_t s) {
-  Batch2Plan plan;
-  hipError_t e = batch2_prepare(items, n, meta, plan, s, true);
-  if (e == hipSuccess) e = batch2_forward(plan, s);
-  const hipError_t r = batch2_release(plan, s);
-  return e != hipSuccess ? e : r;
-}
-
 }  // namespace pjz
