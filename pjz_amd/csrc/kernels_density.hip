@@ -6,11 +6,14 @@
 // exact reverse pass for dL/du.
 //
 // Layout: nb variables of (X, Y), row-major (y fastest), fp32.
-//   forward:  k_density_blur   uf = cone * pad_edge(u)   (LDS tile + halo)
+//   forward:  k_density_conv4<false>  uf = cone * pad_edge(u) (LDS tile +
+//                              halo, four outputs a lane; k_density_blur,
+//                              one output a lane, above half-width 24)
 //             k_density_finish dens, loss from uf         (one pixel a lane)
 //   backward: k_density_gradf  guf = dL/duf               (pixel-local terms
 //                              plus the inflection's neighbour terms)
-//             k_density_corr   gpad = K * guf on the padded domain
+//             k_density_conv4<true> / k_density_corr  gpad = K * guf on
+//                              the padded domain
 //             k_density_fold   gu = gpad folded back through the edge pad
 // The blur is (2r+1)^2 multiply-adds a pixel against 8 bytes of HBM
 // traffic, so at the radii pjz uses it is VALU-bound, not HBM-bound.
@@ -56,6 +59,78 @@ __global__ __launch_bounds__(256) void k_density_blur(const float* __restrict__ 
     for (int c = 0; c < K; ++c) acc = fmaf(w[c], row[c], acc);
   }
   uf[b + (size_t)x * Y + y] = acc;
+}
+
+// The same blur with four outputs a lane along y (16 x 64 outputs a
+// workgroup): a lane slides a K + 3 window along each LDS row of the tile, so
+// each value it reads serves four outputs (~3.4 x fewer LDS reads at K = 9).
+// Cone half-widths up to kDenR4 (the tile and its halo within 64 KB of LDS).
+// Two uses: the forward blur (kPad false: output (x, y) of X x Y reads the
+// edge-clamped input at x - r .. x + r) and the reverse pass's correlation
+// over the padded domain (kPad true: output (p, q) of (X + 2r) x (Y + 2r)
+// reads the input at p - 2r .. p, zero outside it; the cone is symmetric, so
+// the same weights apply).
+constexpr int kDenTileY4 = 64;
+constexpr int kDenR4 = 24;
+
+template <bool kPad>
+__global__ __launch_bounds__(256) void k_density_conv4(const float* __restrict__ u,
+                                                       float* __restrict__ out, int X, int Y,
+                                                       int r, float radius, float inv_sum) {
+  extern __shared__ float sm[];
+  const int SX = kDenTile + 2 * r, SY = kDenTileY4 + 2 * r, K = 2 * r + 1;
+  const int OX = kPad ? X + 2 * r : X, OY = kPad ? Y + 2 * r : Y;
+  const int off = kPad ? 2 * r : r;
+  float* su = sm;            // SX x SY input tile with its halo
+  float* sw = sm + SX * SY;  // K x K normalised cone
+  const float* in = u + (size_t)blockIdx.z * X * Y;
+  const int x0 = blockIdx.x * kDenTile, y0 = blockIdx.y * kDenTileY4;
+  for (int t = threadIdx.x; t < SX * SY; t += blockDim.x) {
+    const int xi = x0 + t / SY - off, yi = y0 + t % SY - off;
+    if (kPad)
+      su[t] = xi >= 0 && xi < X && yi >= 0 && yi < Y ? in[(size_t)xi * Y + yi] : 0.0f;
+    else
+      su[t] = in[(size_t)min(max(xi, 0), X - 1) * Y + min(max(yi, 0), Y - 1)];
+  }
+  for (int t = threadIdx.x; t < K * K; t += blockDim.x)
+    sw[t] = cone_weight(t / K - r, t % K - r, radius) * inv_sum;
+  __syncthreads();
+  const int tx = threadIdx.x / 16, ty = (threadIdx.x % 16) * 4;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= OX || y >= OY) return;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+  for (int a = 0; a < K; ++a) {
+    const float* row = su + (tx + a) * SY + ty;
+    const float* w = sw + a * K;
+    float v0 = row[0], v1 = row[1], v2 = row[2];
+    for (int c = 0; c < K; ++c) {
+      const float v3 = row[c + 3];
+      const float wc = w[c];
+      a0 = fmaf(wc, v0, a0);
+      a1 = fmaf(wc, v1, a1);
+      a2 = fmaf(wc, v2, a2);
+      a3 = fmaf(wc, v3, a3);
+      v0 = v1;
+      v1 = v2;
+      v2 = v3;
+    }
+  }
+  float* o = out + (size_t)blockIdx.z * OX * OY + (size_t)x * OY + y;
+  o[0] = a0;
+  if (y + 1 < OY) o[1] = a1;
+  if (y + 2 < OY) o[2] = a2;
+  if (y + 3 < OY) o[3] = a3;
+}
+
+// Launches k_density_conv4 over an OX x OY output.
+template <bool kPad>
+void launch_conv4(const float* in, float* out, int nb, int X, int Y, int r, float radius,
+                  float inv, hipStream_t s) {
+  const int OX = kPad ? X + 2 * r : X, OY = kPad ? Y + 2 * r : Y;
+  const size_t lds =
+      sizeof(float) * (size_t)((kDenTile + 2 * r) * (kDenTileY4 + 2 * r) + (2 * r + 1) * (2 * r + 1));
+  const dim3 grid((OX + kDenTile - 1) / kDenTile, (OY + kDenTileY4 - 1) / kDenTileY4, nb);
+  hipLaunchKernelGGL(k_density_conv4<kPad>, grid, dim3(256), lds, s, in, out, X, Y, r, radius, inv);
 }
 
 // Per-pixel quantities of the projection and loss at (x, y) of one variable.
@@ -222,9 +297,13 @@ hipError_t density_forward(int nb, int X, int Y, float radius, float alpha, floa
   const int r = density_half_width(radius);
   const float inv = cone_inv_sum(radius, r);
   const int S = kDenTile + 2 * r, K = 2 * r + 1;
-  const size_t lds = sizeof(float) * (size_t)(S * S + K * K);
-  hipLaunchKernelGGL(k_density_blur, den_grid(X, Y, nb), dim3(256), lds, s, u, uf, X, Y, r,
-                     radius, inv);
+  if (r <= kDenR4) {
+    launch_conv4<false>(u, uf, nb, X, Y, r, radius, inv, s);
+  } else {
+    const size_t lds = sizeof(float) * (size_t)(S * S + K * K);
+    hipLaunchKernelGGL(k_density_blur, den_grid(X, Y, nb), dim3(256), lds, s, u, uf, X, Y, r,
+                       radius, inv);
+  }
   hipLaunchKernelGGL(k_density_finish, den_grid(X, Y, nb), dim3(256), 0, s, uf, dens, loss, X, Y,
                      alpha, c, eta, lo, hi);
   return hipGetLastError();
@@ -240,8 +319,11 @@ hipError_t density_backward(int nb, int X, int Y, float radius, float alpha, flo
   const int K = 2 * r + 1;
   hipLaunchKernelGGL(k_density_gradf, den_grid(X, Y, nb), dim3(256), 0, s, uf, gd, gl, guf, X, Y,
                      alpha, c, eta, lo, hi);
-  hipLaunchKernelGGL(k_density_corr, den_grid(X + 2 * r, Y + 2 * r, nb), dim3(256),
-                     sizeof(float) * (size_t)(K * K), s, guf, gpad, X, Y, r, radius, inv);
+  if (r <= kDenR4)
+    launch_conv4<true>(guf, gpad, nb, X, Y, r, radius, inv, s);
+  else
+    hipLaunchKernelGGL(k_density_corr, den_grid(X + 2 * r, Y + 2 * r, nb), dim3(256),
+                       sizeof(float) * (size_t)(K * K), s, guf, gpad, X, Y, r, radius, inv);
   hipLaunchKernelGGL(k_density_fold, den_grid(X, Y, nb), dim3(256), 0, s, gpad, gu, X, Y, r);
   return hipGetLastError();
 }

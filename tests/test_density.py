@@ -151,13 +151,18 @@ def _near_threshold(uf, eta=0.5, tol=1e-5):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape,radius", [((64, 48), 3.5), ((37, 29), 1.0), ((20, 33), 2.5),
-                                          ((40, 40), 6.0), ((1, 17), 2.0), ((2, 9), 3.0)])
+                                          ((40, 40), 6.0), ((1, 17), 2.0), ((2, 9), 3.0),
+                                          ((90, 80), 26.0)])
 def test_density_kernel_gradient(gpu, shape, radius):
   """The HIP reverse pass against torch.autograd through the float64 torch
   restatement on the CPU (whose forward the tests above pin), for random
   cotangents of both outputs; pixels near the threshold excluded."""
   rng = np.random.default_rng(int(10 * radius) + shape[0])
   u = rng.random(shape)
+  if radius > 10:
+    # A wide cone averages noise to ~0.5 everywhere; a ramp keeps most
+    # blurred pixels away from the threshold.
+    u = np.linspace(0, 1, shape[0])[:, None] + 0.2 * (u - 0.5)
   alpha, c = 0.7, 2.0
   gd, gl = rng.standard_normal(shape), rng.standard_normal(shape)
   uc = torch.tensor(u, requires_grad=True)
@@ -191,3 +196,17 @@ def test_density_kernel_rejects_bad_radius(gpu):
     D.density(u, 41.0, 0.5)
   with pytest.raises(ValueError, match="float32"):
     D.density(u.double(), 2.0, 0.5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("radius", [1.0, 24.4, 30.0])
+def test_density_kernel_blur_both_tilings(gpu, radius):
+  """The blur of both tilings (four outputs a lane up to half-width 24, one
+  above) against scipy on the edge-padded variable, with ragged edges."""
+  rng = np.random.default_rng(int(radius * 10))
+  u = rng.random((37, 70))
+  k = D.cone(radius, torch.float64).numpy()
+  r = k.shape[0] // 2
+  ref = scipy.signal.convolve(np.pad(u, r, mode="edge"), k, mode="valid")
+  d, _ = D.density(torch.tensor(u, dtype=torch.float32, device=gpu), radius, 0.0)
+  np.testing.assert_allclose(d.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
