@@ -255,7 +255,8 @@ PJZ_FDTD_API int pjz_fdtd_density_backward(int32_t nb, int32_t xx, int32_t yy, f
  * `grid_end` (zz, 2; column 0 for the x and y components, column 1 for z)
  * -> `eps` (3, xx, yy, zz) by the diagonal sub-pixel average, or the plain
  * volume average when `simple` is non-zero.  fp32 device arrays, last index
- * fastest.  The forward also writes `stats` (12 ll xx yy floats), which the
+ * fastest.  The forward also writes `stats` (12 ll xx yy + 4 ll zz floats:
+ * tile moments and the layers' shares of the cells), which the
  * backward takes back with dL/deps to write dL/dlayers and dL/dpositions
  * (the vector-Jacobian product jax.grad forms through the render; none for
  * the z grids), with a workspace of pjz_fdtd_render_workspace_bytes

@@ -89,7 +89,7 @@ class _RenderKernel(torch.autograd.Function):
     ll, sx, sy = layers.shape
     xx, yy, zz = sx // (2 * m), sy // (2 * m), gs.shape[0]
     eps = torch.empty((3, xx, yy, zz), dtype=layers.dtype, device=layers.device)
-    stats = torch.empty(12 * ll * xx * yy, dtype=layers.dtype, device=layers.device)
+    stats = torch.empty(12 * ll * xx * yy + 4 * ll * zz, dtype=layers.dtype, device=layers.device)
     with torch.cuda.device(layers.device):
       stream = torch.cuda.current_stream(layers.device).cuda_stream
       _lib.check(_lib.load().pjz_fdtd_render_forward(

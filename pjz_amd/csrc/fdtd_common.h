@@ -302,7 +302,7 @@ hipError_t density_backward(int nb, int X, int Y, float radius, float alpha, flo
 // render_stats_floats written by the forward and read by the backward, which
 // needs a workspace of render_workspace_bytes.
 int render_max_layers();
-size_t render_stats_floats(int ll, int xx, int yy);
+size_t render_stats_floats(int ll, int xx, int yy, int zz);
 size_t render_workspace_bytes(int ll, int xx, int yy, int zz);
 hipError_t render_forward(int ll, int m, int xx, int yy, int zz, int simple, const float* layers,
                           const float* pos, const float* gs, const float* ge, float* eps,

@@ -7,19 +7,24 @@
 //                               the inverse and the two in-plane moments of
 //                               the component's half-cell-shifted 2m x 2m
 //                               tile (:12-16, 26-45)
-//             k_render_combine  per (component, x, y, z): the layers' z shares
-//                               of the cell (:47-61) mixed into 1 / (p aoi +
-//                               (1 - p) / avg) or the plain average (:63-99)
+//             k_render_shares   per (z grid, layer, z): the layer's share of
+//                               the cell and its z moment (:47-61, 75-87)
+//             k_render_combine  per (component, x, y, z): the moments mixed
+//                               by the shares into 1 / (p aoi + (1 - p) /
+//                               avg) or the plain average (:63-99)
 //   backward: k_render_bwd_stats   dL/dstats, summed over z
 //             k_render_bwd_share   dL/d(z share, z-moment share), summed over
-//                                  x, y (one workgroup per component and z)
+//                                  chunks of x, y (workgroups per component,
+//                                  z and chunk)
 //             k_render_bwd_pos     dL/d(interface positions)
 //             k_render_bwd_layers  dL/d(layer sub-pixels), gathered through
 //                                  the shifts' edge replication
 // Layouts (fp32): layers (ll, SX = 2m xx, SY = 2m yy); stats (3, 4, ll, xx,
-// yy) with the four moments (mean, mean of inverse, x moment, y moment);
-// eps (3, xx, yy, zz).  Grid bounds gs, ge are (zz, 2): column 0 for the x
+// yy) with the four moments (mean, mean of inverse, x moment, y moment),
+// then each layer's share of each cell (k_render_shares); eps (3, xx, yy,
+// zz).  Grid bounds gs, ge are (zz, 2): column 0 for the x
 // and y components, column 1 for z (:18-24).
+#include <algorithm>
 #include <cmath>
 
 #include "fdtd_common.h"
@@ -95,25 +100,44 @@ __device__ inline Share layer_share(const RenderDims& d, const float* __restrict
   return s;
 }
 
+// The share table: (u, uz) of every layer and cell for both z grids,
+// table[((col * ll + l) * zz + z) * 2 + {0, 1}], after the 12 ll xx yy
+// moments in the stats array.
+__global__ __launch_bounds__(256) void k_render_shares(RenderDims d, const float* __restrict__ pos,
+                                                       const float* __restrict__ gsa,
+                                                       const float* __restrict__ gea,
+                                                       float* __restrict__ tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * d.ll * d.zz) return;
+  const int z = i % d.zz, l = (i / d.zz) % d.ll, col = i / (d.zz * d.ll);
+  const Share sh = layer_share(d, pos, gsa, gea, l, z, col);
+  tab[2 * i] = sh.u;
+  tab[2 * i + 1] = sh.uz;
+}
+
+__device__ inline const float* share_at(const RenderDims& d, const float* tab, int a, int l,
+                                        int z) {
+  return tab + (((a == 2 ? 1 : 0) * d.ll + l) * (size_t)d.zz + z) * 2;
+}
+
 // Mixed sums of one output cell and its value.
 struct Mix {
   float A, B, G[3], out, pii, T;
 };
 
 __device__ inline Mix mix_cell(const RenderDims& d, const float* __restrict__ st,
-                               const float* __restrict__ pos, const float* __restrict__ gsa,
-                               const float* __restrict__ gea, int a, int x, int y, int z) {
+                               const float* __restrict__ tab, int a, int x, int y, int z) {
   Mix r;
   r.A = r.B = r.G[0] = r.G[1] = r.G[2] = 0.0f;
-  const int col = a == 2 ? 1 : 0;
   for (int l = 0; l < d.ll; ++l) {
-    const Share sh = layer_share(d, pos, gsa, gea, l, z, col);
+    const float* sh = share_at(d, tab, a, l, z);
+    const float u = sh[0], uz = sh[1];
     const float avg = st[stat_at(d, a, 0, l, x, y)];
-    r.A += avg * sh.u;
-    r.B += st[stat_at(d, a, 1, l, x, y)] * sh.u;
-    r.G[0] += st[stat_at(d, a, 2, l, x, y)] * sh.u;
-    r.G[1] += st[stat_at(d, a, 3, l, x, y)] * sh.u;
-    r.G[2] += avg * sh.uz;
+    r.A += avg * u;
+    r.B += st[stat_at(d, a, 1, l, x, y)] * u;
+    r.G[0] += st[stat_at(d, a, 2, l, x, y)] * u;
+    r.G[1] += st[stat_at(d, a, 3, l, x, y)] * u;
+    r.G[2] += avg * uz;
   }
   if (d.simple) {
     r.out = r.A;
@@ -129,15 +153,13 @@ __device__ inline Mix mix_cell(const RenderDims& d, const float* __restrict__ st
 }
 
 __global__ __launch_bounds__(256) void k_render_combine(RenderDims d, const float* __restrict__ st,
-                                                        const float* __restrict__ pos,
-                                                        const float* __restrict__ gsa,
-                                                        const float* __restrict__ gea,
+                                                        const float* __restrict__ tab,
                                                         float* __restrict__ eps) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= 3LL * d.xx * d.yy * d.zz) return;
   const int z = (int)(i % d.zz), y = (int)((i / d.zz) % d.yy), x = (int)((i / ((long long)d.zz * d.yy)) % d.xx);
   const int a = (int)(i / ((long long)d.zz * d.yy * d.xx));
-  eps[i] = mix_cell(d, st, pos, gsa, gea, a, x, y, z).out;
+  eps[i] = mix_cell(d, st, tab, a, x, y, z).out;
 }
 
 // Cotangents of a cell's mixed sums A, B, G[0..2] from g = dL/d(out).
@@ -178,29 +200,26 @@ constexpr int kRenderMaxLayers = 16;
 static_assert(kRenderMaxLayers == PJZ_FDTD_RENDER_MAX_LAYERS, "layer bound of the C ABI");
 
 __global__ __launch_bounds__(256) void k_render_bwd_stats(RenderDims d, const float* __restrict__ st,
-                                                          const float* __restrict__ pos,
-                                                          const float* __restrict__ gsa,
-                                                          const float* __restrict__ gea,
+                                                          const float* __restrict__ tab,
                                                           const float* __restrict__ geps,
                                                           float* __restrict__ gst) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= 3LL * d.xx * d.yy) return;
   const int y = (int)(i % d.yy), x = (int)((i / d.yy) % d.xx), a = (int)(i / ((long long)d.yy * d.xx));
-  const int col = a == 2 ? 1 : 0;
   float acc[kRenderMaxLayers][4];
   for (int l = 0; l < kRenderMaxLayers; ++l) acc[l][0] = acc[l][1] = acc[l][2] = acc[l][3] = 0.0f;
   for (int z = 0; z < d.zz; ++z) {
-    const Mix r = mix_cell(d, st, pos, gsa, gea, a, x, y, z);
+    const Mix r = mix_cell(d, st, tab, a, x, y, z);
     const float g = geps[(((size_t)a * d.xx + x) * d.yy + y) * d.zz + z];
     const MixGrad t = mix_grad(d, r, a, g);
 #pragma unroll
     for (int l = 0; l < kRenderMaxLayers; ++l) {
       if (l < d.ll) {
-        const Share sh = layer_share(d, pos, gsa, gea, l, z, col);
-        acc[l][0] += t.gA * sh.u + t.gG[2] * sh.uz;
-        acc[l][1] += t.gB * sh.u;
-        acc[l][2] += t.gG[0] * sh.u;
-        acc[l][3] += t.gG[1] * sh.u;
+        const float* sh = share_at(d, tab, a, l, z);
+        acc[l][0] += t.gA * sh[0] + t.gG[2] * sh[1];
+        acc[l][1] += t.gB * sh[0];
+        acc[l][2] += t.gG[0] * sh[0];
+        acc[l][3] += t.gG[1] * sh[0];
       }
     }
   }
@@ -213,22 +232,27 @@ __global__ __launch_bounds__(256) void k_render_bwd_stats(RenderDims d, const fl
   }
 }
 
-// dL/du and dL/duz (3, ll, zz): one 256-lane workgroup per (component, z)
-// sums the cells of its xy plane.
+// dL/du and dL/duz (3, ll, zz), in parts: workgroup (z, a, ch) sums chunk ch
+// of the cells of its xy plane into part[((ch * 3 + a) * ll + l) * zz + z]
+// (u) and the same after nch * 3 * ll * zz floats (uz); k_render_sum_parts
+// adds the chunks.
+__device__ inline int render_chunks(const RenderDims& d) {
+  return min(max((d.xx * d.yy + 4095) / 4096, 1), 64);
+}
+
 __global__ __launch_bounds__(256) void k_render_bwd_share(RenderDims d, const float* __restrict__ st,
-                                                          const float* __restrict__ pos,
-                                                          const float* __restrict__ gsa,
-                                                          const float* __restrict__ gea,
+                                                          const float* __restrict__ tab,
                                                           const float* __restrict__ geps,
-                                                          float* __restrict__ gu,
-                                                          float* __restrict__ guz) {
+                                                          float* __restrict__ part) {
   __shared__ float red[256];
-  const int z = blockIdx.x, a = blockIdx.y;
+  const int z = blockIdx.x, a = blockIdx.y, ch = blockIdx.z, nch = gridDim.z;
+  const int np = d.xx * d.yy;
+  const int p0 = (int)((long long)ch * np / nch), p1 = (int)((long long)(ch + 1) * np / nch);
   float au[kRenderMaxLayers], auz[kRenderMaxLayers];
   for (int l = 0; l < kRenderMaxLayers; ++l) au[l] = auz[l] = 0.0f;
-  for (int p = threadIdx.x; p < d.xx * d.yy; p += blockDim.x) {
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const int x = p / d.yy, y = p % d.yy;
-    const Mix r = mix_cell(d, st, pos, gsa, gea, a, x, y, z);
+    const Mix r = mix_cell(d, st, tab, a, x, y, z);
     const MixGrad t = mix_grad(d, r, a, geps[(((size_t)a * d.xx + x) * d.yy + y) * d.zz + z]);
 #pragma unroll
     for (int l = 0; l < kRenderMaxLayers; ++l) {
@@ -252,7 +276,9 @@ __global__ __launch_bounds__(256) void k_render_bwd_share(RenderDims d, const fl
           if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
           __syncthreads();
         }
-        if (threadIdx.x == 0) (w == 0 ? gu : guz)[((size_t)a * d.ll + l) * d.zz + z] = red[0];
+        if (threadIdx.x == 0)
+          part[(size_t)w * nch * 3 * d.ll * d.zz + (((size_t)ch * 3 + a) * d.ll + l) * d.zz + z] =
+              red[0];
         __syncthreads();
       }
     }
@@ -270,12 +296,30 @@ __device__ inline float dclip(float v, float lo, float hi) {
 // dL/dpos (ll - 1): interface j is layer j's upper bound and layer j+1's
 // lower bound.  One workgroup; lane j of the first ll - 1 sums over
 // components and z.
+// Adds the chunks of k_render_bwd_share: gus[w][a][l][z] (w = 0: dL/du,
+// 1: dL/duz), one lane per entry.
+__global__ __launch_bounds__(256) void k_render_sum_parts(RenderDims d, const float* __restrict__ part,
+                                                          float* __restrict__ gus) {
+  const int n = 3 * d.ll * d.zz;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * n) return;
+  const int w = i / n, e = i % n;
+  const int nch = render_chunks(d);
+  float t = 0.0f;
+  for (int ch = 0; ch < nch; ++ch) t += part[((size_t)w * nch + ch) * n + e];
+  gus[i] = t;
+}
+
 __global__ __launch_bounds__(64) void k_render_bwd_pos(RenderDims d, const float* __restrict__ pos,
                                                        const float* __restrict__ gsa,
                                                        const float* __restrict__ gea,
-                                                       const float* __restrict__ gu,
-                                                       const float* __restrict__ guz,
+                                                       const float* __restrict__ gus,
                                                        float* __restrict__ gpos) {
+  const size_t n = (size_t)3 * d.ll * d.zz;
+  // dL/du (w = 0) and dL/duz (w = 1) of layer l at (a, z).
+  auto sum = [&](int w, int a, int l, int z) {
+    return gus[w * n + ((size_t)a * d.ll + l) * d.zz + z];
+  };
   for (int j = threadIdx.x; j < d.ll - 1; j += blockDim.x) {
     float acc = 0.0f;
     for (int a = 0; a < 3; ++a) {
@@ -287,15 +331,13 @@ __global__ __launch_bounds__(64) void k_render_bwd_pos(RenderDims d, const float
         {
           const Share sh = layer_share(d, pos, gsa, gea, j, z, col);
           const float dp1 = dclip(pos[j], gs, ge);
-          const size_t o = ((size_t)a * d.ll + j) * d.zz + z;
-          acc += (gu[o] / dz + guz[o] * k * (sh.p1 - c)) * dp1;
+          acc += (sum(0, a, j, z) / dz + sum(1, a, j, z) * k * (sh.p1 - c)) * dp1;
         }
         // layer j + 1: lo = pos[j] -> p0
         {
           const Share sh = layer_share(d, pos, gsa, gea, j + 1, z, col);
           const float dp0 = dclip(pos[j], gs, ge);
-          const size_t o = ((size_t)a * d.ll + j + 1) * d.zz + z;
-          acc += (-gu[o] / dz + guz[o] * k * (c - sh.p0)) * dp0;
+          acc += (-sum(0, a, j + 1, z) / dz + sum(1, a, j + 1, z) * k * (c - sh.p0)) * dp0;
         }
       }
     }
@@ -361,21 +403,31 @@ unsigned blocks_for(long long n, int threads = 256) { return (unsigned)((n + thr
 
 int render_max_layers() { return kRenderMaxLayers; }
 
-size_t render_stats_floats(int ll, int xx, int yy) { return (size_t)12 * ll * xx * yy; }
+size_t render_stats_floats(int ll, int xx, int yy, int zz) {
+  return (size_t)12 * ll * xx * yy + (size_t)4 * ll * zz;
+}
+
+static int render_chunks_host(int xx, int yy) {
+  return std::min(std::max((xx * yy + 4095) / 4096, 1), 64);
+}
 
 size_t render_workspace_bytes(int ll, int xx, int yy, int zz) {
-  return align256(sizeof(float) * render_stats_floats(ll, xx, yy)) +
-         2 * align256(sizeof(float) * (size_t)3 * ll * zz);
+  return align256(sizeof(float) * (size_t)12 * ll * xx * yy) +
+         align256(sizeof(float) * (size_t)2 * render_chunks_host(xx, yy) * 3 * ll * zz) +
+         align256(sizeof(float) * (size_t)2 * 3 * ll * zz);
 }
 
 hipError_t render_forward(int ll, int m, int xx, int yy, int zz, int simple, const float* layers,
                           const float* pos, const float* gs, const float* ge, float* eps,
                           float* stats, hipStream_t s) {
   const RenderDims d{ll, m, xx, yy, zz, simple};
+  float* tab = stats + (size_t)12 * ll * xx * yy;
   hipLaunchKernelGGL(k_render_stats, dim3(blocks_for(3LL * ll * xx * yy)), dim3(256), 0, s, d,
                      layers, stats);
+  hipLaunchKernelGGL(k_render_shares, dim3(blocks_for(2LL * ll * zz)), dim3(256), 0, s, d, pos,
+                     gs, ge, tab);
   hipLaunchKernelGGL(k_render_combine, dim3(blocks_for(3LL * xx * yy * zz)), dim3(256), 0, s, d,
-                     stats, pos, gs, ge, eps);
+                     stats, tab, eps);
   return hipGetLastError();
 }
 
@@ -384,17 +436,21 @@ hipError_t render_backward(int ll, int m, int xx, int yy, int zz, int simple, co
                            const float* stats, const float* g_eps, float* g_layers, float* g_pos,
                            char* ws, hipStream_t s) {
   const RenderDims d{ll, m, xx, yy, zz, simple};
+  const float* tab = stats + (size_t)12 * ll * xx * yy;
   float* gst = (float*)ws;
-  float* gu = (float*)(ws + align256(sizeof(float) * render_stats_floats(ll, xx, yy)));
-  float* guz = (float*)((char*)gu + align256(sizeof(float) * (size_t)3 * ll * zz));
+  float* part = (float*)(ws + align256(sizeof(float) * (size_t)12 * ll * xx * yy));
+  float* gus = (float*)((char*)part + align256(sizeof(float) * (size_t)2 *
+                                               render_chunks_host(xx, yy) * 3 * ll * zz));
   hipLaunchKernelGGL(k_render_bwd_stats, dim3(blocks_for(3LL * xx * yy)), dim3(256), 0, s, d,
-                     stats, pos, gs, ge, g_eps, gst);
+                     stats, tab, g_eps, gst);
   hipLaunchKernelGGL(k_render_bwd_layers, dim3(blocks_for((long long)ll * 4 * m * m * xx * yy)),
                      dim3(256), 0, s, d, layers, gst, g_layers);
   if (ll > 1 && g_pos) {
-    hipLaunchKernelGGL(k_render_bwd_share, dim3(zz, 3), dim3(256), 0, s, d, stats, pos, gs, ge,
-                       g_eps, gu, guz);
-    hipLaunchKernelGGL(k_render_bwd_pos, dim3(1), dim3(64), 0, s, d, pos, gs, ge, gu, guz, g_pos);
+    hipLaunchKernelGGL(k_render_bwd_share, dim3(zz, 3, render_chunks_host(xx, yy)), dim3(256),
+                       0, s, d, stats, tab, g_eps, part);
+    hipLaunchKernelGGL(k_render_sum_parts, dim3(blocks_for(6LL * ll * zz)), dim3(256), 0, s, d,
+                       part, gus);
+    hipLaunchKernelGGL(k_render_bwd_pos, dim3(1), dim3(64), 0, s, d, pos, gs, ge, gus, g_pos);
   }
   return hipGetLastError();
 }
