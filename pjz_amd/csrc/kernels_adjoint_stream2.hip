@@ -43,6 +43,7 @@ struct Adj2Args {
   const float* mask;
   int X, Y, Z, lo, hi, NP, N, P;
   float dt;
+  int pair;  // CPL = 2 with even PML widths and Z: psi entries in aligned pairs
   int LX, nty, ntiles;
   int nsteps;  // steps of the simulation (batched runs)
 };
@@ -168,14 +169,36 @@ __device__ inline void adj2_fetch(Adj2Plane<CPL>& P, const Adj2Args& a, int x, i
     load_col<CPL, true>(P.kuc[1], a.cb + (2 * N + o + Z), zb, Z, v);
   }
   const int pb = (x * Y + yc) * a.NP;
+  if (CPL == 2 && a.pair) {
+    // A lane's two cells are both PML or both not, with consecutive, 8-byte
+    // aligned psi entries: one 64-bit load per component.
+    const bool ph = zc.pml[0] && active;
+    const int pi = pb + zc.pidx[0];
+    float2 u0 = make_float2(0.0f, 0.0f), u1 = u0, p0 = u0, p1 = u0;
+    if (ph) {
+      u0 = *reinterpret_cast<const float2*>(a.luin + pi);
+      u1 = *reinterpret_cast<const float2*>(a.luin + (a.P + pi));
+      p0 = *reinterpret_cast<const float2*>(a.lpin + pi);
+      p1 = *reinterpret_cast<const float2*>(a.lpin + (a.P + pi));
+    }
+    P.lu[0][0] = u0.x;
+    P.lu[0][CPL - 1] = u0.y;
+    P.lu[1][0] = u1.x;
+    P.lu[1][CPL - 1] = u1.y;
+    P.lp[0][0] = p0.x;
+    P.lp[0][CPL - 1] = p0.y;
+    P.lp[1][0] = p1.x;
+    P.lp[1][CPL - 1] = p1.y;
+  } else {
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const bool ph = zc.pml[c] && active;
-    const int pi = pb + zc.pidx[c];
-    P.lu[0][c] = ph ? a.luin[pi] : 0.0f;
-    P.lu[1][c] = ph ? a.luin[a.P + pi] : 0.0f;
-    P.lp[0][c] = ph ? a.lpin[pi] : 0.0f;
-    P.lp[1][c] = ph ? a.lpin[a.P + pi] : 0.0f;
+    for (int c = 0; c < CPL; ++c) {
+      const bool ph = zc.pml[c] && active;
+      const int pi = pb + zc.pidx[c];
+      P.lu[0][c] = ph ? a.luin[pi] : 0.0f;
+      P.lu[1][c] = ph ? a.luin[a.P + pi] : 0.0f;
+      P.lp[0][c] = ph ? a.lpin[pi] : 0.0f;
+      P.lp[1][c] = ph ? a.lpin[a.P + pi] : 0.0f;
+    }
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k)
@@ -337,14 +360,24 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
       store_col<CPL, true>(a.G + (k * N + o), gn, zb, Z);
     }
     const int pb = ((x - 1) * Y + y) * a.NP;
+    if (CPL == 2 && a.pair) {
+      if (zc.pml[0]) {
+        const int pi = pb + zc.pidx[0];
+        *reinterpret_cast<float2*>(a.luout + pi) = make_float2(lu2[0][0], lu2[0][CPL - 1]);
+        *reinterpret_cast<float2*>(a.luout + a.P + pi) = make_float2(lu2[1][0], lu2[1][CPL - 1]);
+        *reinterpret_cast<float2*>(a.lpout + pi) = make_float2(lp2[0][0], lp2[0][CPL - 1]);
+        *reinterpret_cast<float2*>(a.lpout + a.P + pi) = make_float2(lp2[1][0], lp2[1][CPL - 1]);
+      }
+    } else {
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      if (zc.pml[c]) {
-        const int pi = pb + zc.pidx[c];
-        a.luout[pi] = lu2[0][c];
-        a.luout[a.P + pi] = lu2[1][c];
-        a.lpout[pi] = lp2[0][c];
-        a.lpout[a.P + pi] = lp2[1][c];
+      for (int c = 0; c < CPL; ++c) {
+        if (zc.pml[c]) {
+          const int pi = pb + zc.pidx[c];
+          a.luout[pi] = lu2[0][c];
+          a.luout[a.P + pi] = lu2[1][c];
+          a.lpout[pi] = lp2[0][c];
+          a.lpout[a.P + pi] = lp2[1][c];
+        }
       }
     }
   }
@@ -552,6 +585,7 @@ static Adj2Args adj2_args(const Geom& g, const pjz_fdtd_problem& p, const float*
   a.N = (int)g.N;
   a.P = g.X * g.Y * g.NP;
   a.dt = p.dt;
+  a.pair = g.Z > 64 && g.Z <= 128 && g.Z % 2 == 0 && g.lo % 2 == 0 && g.hi % 2 == 0;
   a.nty = (g.Y + kAdj2WY - 1) / kAdj2WY;
   if (LX <= 0) {
     const int slots =
