@@ -132,6 +132,12 @@ struct Adj2Plane {
   float kuc[2][CPL];  // cb x, z at (x, y+1): lK = kuc kue, formed where used so
                       // the fetch does not wait for its own loads
   float sig[3];       // absorber at (x, y)
+  // Owned rows, with DG: D^n at x (level 1), G and D^(n-1) at x-1 (level
+  // 2), fetched with the rest one plane ahead, so the stores of phase C wait
+  // for no load.  (Loaded in the plane itself, every store waits for the
+  // whole prefetch too: the loads' count is path-dependent, so the wait is
+  // vmcnt(0).)  Unused without DG.
+  float d1[3][CPL], g[3][CPL], d2[3][CPL];
 };
 
 // Carried from plane x-1 to plane x.
@@ -149,12 +155,35 @@ struct Adj2Carry {
   float sigm[3];
 };
 
+// D^n at x where lvl1, G and D^(n-1) at x-1 where lvl2 (owned rows).
 template <int CPL>
+__device__ inline void adj2_load_dg(float (&d1)[3][CPL], float (&g)[3][CPL], float (&d2)[3][CPL],
+                                    const Adj2Args& a, int x, int yc, int zb, bool lvl1,
+                                    bool lvl2) {
+  const int N = a.N, Z = a.Z;
+  const int o = (x * a.Y + yc) * Z, om = o - a.Y * Z;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    load_col<CPL, true>(d1[k], a.D1 + (k * N + o), zb, Z, lvl1);
+    load_col<CPL, true>(g[k], a.G + (k * N + om), zb, Z, lvl2);
+    load_col<CPL, true>(d2[k], a.D2 + (k * N + om), zb, Z, lvl2);
+  }
+}
+
+// DG: also fetch the owned rows' D and G (one plane ahead).  That takes 27
+// more registers a lane at two cells a lane; the batch kernel, whose one-cell
+// form would then pass 128 registers and lose a quarter of its waves, loads
+// them in the plane instead.
+template <int CPL, bool DG>
 __device__ inline void adj2_fetch(Adj2Plane<CPL>& P, const Adj2Args& a, int x, int yc, bool top,
-                                  int zb, bool active, const AdjCoef<CPL>& zc) {
+                                  int zb, bool active, const AdjCoef<CPL>& zc, bool own,
+                                  bool row_valid, int xa) {
   const int N = a.N, Y = a.Y, Z = a.Z;
   const int o = (x * Y + yc) * Z;
   const bool nx = active && x + 1 < a.X;
+  if (DG)
+    adj2_load_dg<CPL>(P.d1, P.g, P.d2, a, x, yc, zb, own && active,
+                      own && row_valid && x >= xa && x >= 1);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     load_col<CPL, true>(P.le[k], a.lEin + (k * N + o + Y * Z), zb, Z, nx);
@@ -214,7 +243,7 @@ struct Adj2Lds {
   float sH2[WY + 3][2][64 * CPL];     // level-2 lH x, z of plane x-1
 };
 
-template <int CPL, int WY>
+template <int CPL, int WY, bool DG>
 __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& L,
                                            Adj2Carry<CPL>& s, const Adj2Plane<CPL>& cur,
                                            Adj2Plane<CPL>& nxt, const AdjCoef<CPL>& zc, int x,
@@ -230,21 +259,25 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
   const int N = a.N, Y = a.Y, Z = a.Z;
   const float dt = a.dt;
   // This plane's D^n (level 1 at x) and G, D^(n-1) (level 2 at x-1), owned
-  // rows only: issued now, used in phases B and C.
+  // rows only, used in phases B and C.  Without DG they are requested before
+  // the next plane's inputs: waiting for them (in-order completion) then
+  // leaves the prefetch in flight.
   float d1[3][CPL], g[3][CPL], d2[3][CPL];
-  {
-    const int o = (x * Y + yc) * Z, om = ((x - 1) * Y + yc) * Z;
-    const bool st = own && row_valid && l2;
+  if (DG) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      load_col<CPL, true>(d1[k], a.D1 + (k * N + o), zb, Z, own && act);
-      load_col<CPL, true>(g[k], a.G + (k * N + om), zb, Z, st);
-      load_col<CPL, true>(d2[k], a.D2 + (k * N + om), zb, Z, st);
-    }
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        d1[k][c] = cur.d1[k][c];
+        g[k][c] = cur.g[k][c];
+        d2[k][c] = cur.d2[k][c];
+      }
+  } else {
+    adj2_load_dg<CPL>(d1, g, d2, a, x, yc, zb, own && act, own && row_valid && l2);
   }
-  // The next plane's inputs are requested after these: waiting for D and G
-  // (in-order completion) then leaves the prefetch in flight.
-  if (x + 1 <= xb) adj2_fetch<CPL>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, zc);
+  if (x + 1 <= xb)
+    adj2_fetch<CPL, DG>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, zc, own,
+                        row_valid, xa);
 
   // ---- A: level-1 lH at plane x
   float K1[3][CPL];
@@ -354,7 +387,8 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
     for (int k = 0; k < 3; ++k) {
       float gn[CPL];
 #pragma unroll
-      for (int c = 0; c < CPL; ++c) gn[c] = g[k][c] + s.gpm[k][c] + s.e1m[k][c] * d2[k][c];
+      for (int c = 0; c < CPL; ++c)
+        gn[c] = g[k][c] + s.gpm[k][c] + s.e1m[k][c] * d2[k][c];
       store_col<CPL, true>(a.lEout + (k * N + o), e2[k], zb, Z);
       store_col<CPL, true>(a.lHout + (k * N + o), h2[k], zb, Z);
       store_col<CPL, true>(a.G + (k * N + o), gn, zb, Z);
@@ -406,7 +440,7 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
 }
 
 // One tile (WY rows x LX planes) of one pair of reversed steps.
-template <int CPL, int WY>
+template <int CPL, int WY, bool DG>
 __device__ __forceinline__ void adj2_tile(const Adj2Args& a, int tile, Adj2Lds<CPL, WY>& L) {
   constexpr int NW = WY + 3;
   const int Z = a.Z;
@@ -456,7 +490,8 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args& a, int tile, Adj2Lds<C
     }
   }
   Adj2Plane<CPL> cur, nxt;
-  adj2_fetch<CPL>(cur, a, xs, yc, w == NW - 1, zb, row_valid, zc);
+  adj2_fetch<CPL, DG>(cur, a, xs, yc, w == NW - 1, zb, row_valid, zc, w >= 2 && w <= WY + 1,
+                      row_valid, xa);
   // lK x, z of the first plane for the row below.
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
@@ -466,7 +501,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args& a, int tile, Adj2Lds<C
   lds_barrier();
   int par = 0;
   for (int x = xs; x <= xb; ++x) {
-    adj2_plane<CPL, WY>(a, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid, par);
+    adj2_plane<CPL, WY, DG>(a, L, s, cur, nxt, zc, x, xa, xb, w, y, yc, lane, zb, row_valid, par);
     cur = nxt;
     par ^= 1;
   }
@@ -478,7 +513,7 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2(Adj2Args a) {
   // Tiles adjacent in y (sharing ghost rows) go to the same XCD.
   int tile = blockIdx.x;
   if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
-  adj2_tile<CPL, WY>(a, tile, L);
+  adj2_tile<CPL, WY, true>(a, tile, L);
 }
 
 // Batched pairs of reversed steps: persistent workgroups take (simulation,
@@ -511,7 +546,7 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2_batch(
     const long long f3 = 3LL * a.N;
     a.D1 += (long long)(step - seg_begin) * f3;
     a.D2 = a.D1 - f3;
-    adj2_tile<CPL, WY>(a, t - tile_start[lo], L);
+    adj2_tile<CPL, WY, false>(a, t - tile_start[lo], L);
   }
 }
 
