@@ -31,18 +31,6 @@
 
 namespace pjz {
 
-// State carried from plane x-1 to plane x.
-template <int CPL>
-struct Carry2 {
-  float e0[3][CPL];   // input E at x
-  float e1m[3][CPL];  // level-1 E at x-1
-  float h1m[3][CPL];  // level-1 H at x-1
-  float cbm[3][CPL];  // cb at x-1
-  float psm[4][CPL];  // level-1 psi at x-1
-  float h2p[2][CPL];  // level-2 (Hy, Hz) at x-2
-  float sigm[3];
-};
-
 // Row exchange buffers; a row's cell c of lane l (of the LPR = 64 / RPW
 // lanes that hold the row) sits at [c * LPR + l], so the lanes of an access
 // hit consecutive banks.
@@ -482,6 +470,9 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
     LX = grid_chunk_x(g.X, (g.Y + WY - 1) / WY, device_cus(), 2);
   }
   Stream2Args a = make_args2(g, p, mask, src, win, ws, L, WY, LX, 0);
+  // 8-row tiles at two cells per lane take their inputs through an LDS ring
+  // two planes deep where the height and PML widths allow.
+  const bool ring = cpl == 2 && WY == 8 && stream2_ring_supported(g);
   Stream2Rare* rare = (Stream2Rare*)(ws + L.rare);
   hipLaunchKernelGGL(k_put_rare, dim3(1), dim3(1), 0, s, rare,
                      rare_of(p, win, r.capture, r.begin));
@@ -513,6 +504,7 @@ hipError_t run_stream2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
     else if (WY == 4) launch2<2, 4, true>(a, s);
     else if (WY == 12) launch2<2, 12, true>(a, s);
     else if (WY == 13) launch2<2, 13, true>(a, s);
+    else if (ring) (void)launch_stream2_ring(a, s);
     else launch2<2, 8, true>(a, s);
     *r.cur = 1 - c;
   }

@@ -197,6 +197,18 @@ __device__ inline void e_row(float (&en)[3][CPL], float (&psEn)[2][CPL], const f
   }
 }
 
+// State of the two-step kernels carried from plane x-1 to plane x.
+template <int CPL>
+struct Carry2 {
+  float e0[3][CPL];   // input E at x
+  float e1m[3][CPL];  // level-1 E at x-1
+  float h1m[3][CPL];  // level-1 H at x-1
+  float cbm[3][CPL];  // cb at x-1
+  float psm[4][CPL];  // level-1 psi at x-1
+  float h2p[2][CPL];  // level-2 (Hy, Hz) at x-2
+  float sigm[3];
+};
+
 // Level-1 inputs of one row at plane x, fetched one plane ahead.
 template <int CPL>
 struct Plane2 {
@@ -331,6 +343,11 @@ __global__ void k_put_rare(Stream2Rare* dst, Stream2Rare v);
 Stream2Args make_args2(const Geom& g, const pjz_fdtd_problem& p, const float* mask,
                        const Src& src, const OutWin& win, char* ws, const Layout& L, int WY,
                        int LX, int cur);
+
+// Two-step kernel at two cells per lane with its inputs staged through an
+// LDS ring (kernels_stream2_ring.hip): the shapes it takes, and one launch.
+bool stream2_ring_supported(const Geom& g);
+hipError_t launch_stream2_ring(const Stream2Args& a, hipStream_t s);
 
 // Batched three-step launches at one cell per lane (kernels_stream_deep.hip):
 // rows per tile, workgroups resident per CU, and launch t's persistent grid
