@@ -216,6 +216,7 @@ struct Batch2Plan {
   int wy[2];                        // per class: rows per tile
   int depth0, nlaunch0;  // zz <= 64 class: steps per launch (2, or 3 in
                          // forward-only batches) and its three-step launches
+  bool ring1;            // zz > 64 class: every simulation fits the LDS-ring kernel
   char* meta;
   size_t args_bytes, rare_bytes, ts_bytes;
 };

@@ -643,9 +643,11 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
   plan.graph = nullptr;
   plan.exec = nullptr;
   plan.chunk = std::min(std::max(items[0].p->graph_steps / 2, 0), kMaxChunkPairs);
+  plan.ring1 = true;
   for (int i = 0; i < n; ++i) {
     if (!batch2_supported(items[i].g)) return hipErrorInvalidValue;
     cls[items[i].g.Z <= 64 ? 0 : 1].push_back(i);
+    if (items[i].g.Z > 64) plan.ring1 = plan.ring1 && stream2_ring_supported(items[i].g);
     plan.npairs = std::max(plan.npairs, (steps_of(*items[i].p) + 1) / 2);
   }
   // Host image of the argument views, launch data and tile prefix sums,
@@ -715,7 +717,10 @@ hipError_t batch2_prepare(const BatchItem* items, int n, char* meta, Batch2Plan&
         }
       }
     }
-    const int cLX = c == 1            ? chunk(kBatchWY2, batch2_resident<2, kBatchWY2>(), 2)
+    const int cLX = c == 1 ? chunk(kBatchWY2,
+                                   plan.ring1 ? stream2_ring_batch_resident()
+                                              : batch2_resident<2, kBatchWY2>(),
+                                   2)
                     : plan.depth0 == 3 ? chunk(WY, deep_batch_resident(), 3)
                     : WY == kBatchWY1Small ? chunk(WY, batch2_resident<1, kBatchWY1Small>(), 2)
                                            : chunk(WY, batch2_resident<1, kBatchWY1>(), 2);
@@ -764,6 +769,10 @@ static void launch_pair(const Batch2Plan& plan, hipStream_t s, int* ctr, const i
       hipLaunchKernelGGL((k_stream2_batch<1, kBatchWY1, true>), dim3(grid),
                          dim3(64 * (kBatchWY1 + 3)), 0, s, a, ts, nc, plan.total[c], ctr + c * kQueues,
                          step_base, step_off);
+    } else if (plan.ring1) {
+      const int grid = std::min(plan.total[c], plan.cus * stream2_ring_batch_resident());
+      launch_stream2_ring_batch(a, ts, nc, plan.total[c], ctr + c * kQueues, step_base, step_off,
+                                grid, s);
     } else {
       const int grid = std::min(plan.total[c], plan.cus * batch2_resident<2, kBatchWY2>());
       hipLaunchKernelGGL((k_stream2_batch<2, kBatchWY2, true>), dim3(grid),

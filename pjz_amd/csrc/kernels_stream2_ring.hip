@@ -337,13 +337,10 @@ __device__ inline void ring_sig(float (&sig)[3], const Stream2Args& a, int x, in
   for (int k = 0; k < 3; ++k) sig[k] = ok ? uniform_load(a.mask, (k * a.X + xm) * a.Y + yc) : 0.0f;
 }
 
+// One tile (WY rows x LX planes) of one simulation for steps a.n, a.n+1.
 template <int WY>
-__global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_ring(Stream2Args a) {
+__device__ __forceinline__ void ring_tile(const Stream2Args& a, int tile, RingLds<WY>& L) {
   constexpr int NW = WY + 3;
-  __shared__ RingLds<WY> L;
-  // Tiles adjacent in y (which share ghost rows) run on the same XCD.
-  int tile = blockIdx.x;
-  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
   const int Z = a.Z;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -410,6 +407,52 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_ring(Stream2Args a) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) cur.sig[k] = nsig[k];
   }
+  // The last planes' copies (repeats of the last plane, never read) land
+  // before the ring is reused or the workgroup ends.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <int WY>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_ring(Stream2Args a) {
+  __shared__ RingLds<WY> L;
+  // Tiles adjacent in y (which share ghost rows) run on the same XCD.
+  int tile = blockIdx.x;
+  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
+  ring_tile<WY>(a, tile, L);
+}
+
+// Batched form: persistent workgroups take (simulation, tile) items from the
+// per-XCD queues, as k_stream2_batch does (args: two views per simulation,
+// selected by the parity of the step pair).
+template <int WY>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_stream2_ring_batch(
+    const Stream2Args* __restrict__ args, const int* __restrict__ tile_start, int nsims,
+    int total, int* __restrict__ counter, const int* __restrict__ step_base, int step_off) {
+  __shared__ RingLds<WY> L;
+  __shared__ int s_item, s_next;
+  if (threadIdx.x == 0) s_next = 0;
+  const int step =
+      (step_base ? *(const __attribute__((address_space(4))) int*)step_base : 0) + step_off;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = pop_item(counter, total, s_next);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (t >= total) break;
+    int lo = 0, hi = nsims - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    Stream2Args a = args[2 * lo + ((step >> 1) & 1)];
+    if (step >= a.nsteps) continue;
+    a.n = step;
+    a.flags = (out_slot(a.rare, step) >= 0 ? SNAP1 : 0) |
+              (out_slot(a.rare, step + 1) >= 0 ? SNAP2 : 0) |
+              (((rare_cptr)a.rare)->cap ? CAPTURE : 0);
+    ring_tile<WY>(a, t - tile_start[lo], L);
+  }
 }
 
 }  // namespace
@@ -420,10 +463,30 @@ bool stream2_ring_supported(const Geom& g) {
   return fits_rsrc(g) && g.Z > 64 && g.Z <= 128 && g.Z % 4 == 0 && g.NP <= kRingPsi;
 }
 
+static constexpr int kRingWY = 8;
+
 hipError_t launch_stream2_ring(const Stream2Args& a, hipStream_t s) {
-  constexpr int WY = 8;
-  hipLaunchKernelGGL(k_stream2_ring<WY>, dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
+  hipLaunchKernelGGL(k_stream2_ring<kRingWY>, dim3(a.ntiles), dim3(64 * (kRingWY + 3)), 0, s, a);
   return hipGetLastError();
+}
+
+int stream2_ring_batch_resident() {
+  static const int n = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &b, reinterpret_cast<const void*>(&k_stream2_ring_batch<kRingWY>), 64 * (kRingWY + 3),
+            0) != hipSuccess || b < 1)
+      b = 1;
+    return b;
+  }();
+  return n;
+}
+
+void launch_stream2_ring_batch(const Stream2Args* args, const int* tile_start, int nsims,
+                               int total, int* counter, const int* step_base, int step_off,
+                               int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_stream2_ring_batch<kRingWY>, dim3(grid), dim3(64 * (kRingWY + 3)), 0, s,
+                     args, tile_start, nsims, total, counter, step_base, step_off);
 }
 
 }  // namespace pjz

@@ -45,8 +45,15 @@ struct Stream2Args {
   int nsteps;  // steps of the simulation (batched runs)
 };
 
+// The block read through the constant address space: wave-uniform scalar
+// loads, which a wait for vector memory operations does not include (the
+// LDS-ring kernel counts those itself, and a vector load here would make
+// every capturing plane wait for all of them).
+typedef const __attribute__((address_space(4))) Stream2Rare* rare_cptr;
+
 // Snapshot slot of a step in the output schedule, or -1.
-__device__ inline int out_slot(const Stream2Rare* r, int step) {
+__device__ inline int out_slot(const Stream2Rare* rg, int step) {
+  const rare_cptr r = (rare_cptr)rg;
   if (!r->out || step < r->ostart || step >= r->ostop || (step - r->ostart) % r->ostep != 0)
     return -1;
   return (step - r->ostart) / r->ostep;
@@ -286,9 +293,9 @@ template <int CPL, bool vec>
 __device__ __forceinline__ void store_rare(const Stream2Args& a, bool snap, bool cap, int step, int x,
                                         int y, int zb, const float (&en)[3][CPL],
                                         const float (&eo)[3][CPL]) {
-  const Stream2Rare* r = a.rare;
+  const rare_cptr r = (rare_cptr)a.rare;
   if (snap) {
-    const int slot = out_slot(r, step);
+    const int slot = out_slot(a.rare, step);
     if (slot >= 0 && x >= r->wx0 && x < r->wx0 + r->wex && y >= r->wy0 && y < r->wy0 + r->wey) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -348,6 +355,12 @@ Stream2Args make_args2(const Geom& g, const pjz_fdtd_problem& p, const float* ma
 // LDS ring (kernels_stream2_ring.hip): the shapes it takes, and one launch.
 bool stream2_ring_supported(const Geom& g);
 hipError_t launch_stream2_ring(const Stream2Args& a, hipStream_t s);
+// Its batched form (8-row tiles): workgroups resident per CU, and one launch
+// of `grid` persistent workgroups over the class's items.
+int stream2_ring_batch_resident();
+void launch_stream2_ring_batch(const Stream2Args* args, const int* tile_start, int nsims,
+                               int total, int* counter, const int* step_base, int step_off,
+                               int grid, hipStream_t s);
 
 // Batched three-step launches at one cell per lane (kernels_stream_deep.hip):
 // rows per tile, workgroups resident per CU, and launch t's persistent grid
