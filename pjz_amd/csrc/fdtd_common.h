@@ -267,6 +267,7 @@ struct AdjBatchPlan {
   int count[2], base[2], total[2];
   char* meta;
   size_t args_bytes, ts_bytes, ls_bytes;
+  bool ring1;  // two-step reverse: the zz > 64 class fits the LDS-ring kernel
 };
 size_t adjoint_batch_meta_bytes(int n, int S);
 hipError_t adjoint_batch_prepare(const BatchItem* items, int n, int S, const int* lstart,

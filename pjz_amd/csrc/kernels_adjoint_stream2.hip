@@ -155,6 +155,15 @@ struct Adj2Carry {
   float sigm[3];
 };
 
+// Absorber values of row yc at plane x where ok (wave-uniform scalar loads;
+// the plane index is clamped so that a load issued ahead of its condition
+// stays in the array).
+__device__ inline void adj2_sig(float (&sig)[3], const Adj2Args& a, int x, int yc, bool ok) {
+  const int xm = min(x, a.X - 1);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) sig[k] = ok ? uniform_load(a.mask, (k * a.X + xm) * a.Y + yc) : 0.0f;
+}
+
 // D^n at x where lvl1, G and D^(n-1) at x-1 where lvl2 (owned rows).
 template <int CPL>
 __device__ inline void adj2_load_dg(float (&d1)[3][CPL], float (&g)[3][CPL], float (&d2)[3][CPL],
@@ -205,10 +214,16 @@ __device__ inline void adj2_fetch(Adj2Plane<CPL>& P, const Adj2Args& a, int x, i
     const int pi = pb + zc.pidx[0];
     float2 u0 = make_float2(0.0f, 0.0f), u1 = u0, p0 = u0, p1 = u0;
     if (ph) {
-      u0 = *reinterpret_cast<const float2*>(a.luin + pi);
-      u1 = *reinterpret_cast<const float2*>(a.luin + (a.P + pi));
-      p0 = *reinterpret_cast<const float2*>(a.lpin + pi);
-      p1 = *reinterpret_cast<const float2*>(a.lpin + (a.P + pi));
+      gcfloat* lu = to_global(a.luin);
+      gcfloat* lp = to_global(a.lpin);
+      const f32x2 a0 = *reinterpret_cast<gcf32x2*>(lu + pi);
+      const f32x2 a1 = *reinterpret_cast<gcf32x2*>(lu + (a.P + pi));
+      const f32x2 b0 = *reinterpret_cast<gcf32x2*>(lp + pi);
+      const f32x2 b1 = *reinterpret_cast<gcf32x2*>(lp + (a.P + pi));
+      u0 = make_float2(a0.x, a0.y);
+      u1 = make_float2(a1.x, a1.y);
+      p0 = make_float2(b0.x, b0.y);
+      p1 = make_float2(b1.x, b1.y);
     }
     P.lu[0][0] = u0.x;
     P.lu[0][CPL - 1] = u0.y;
@@ -223,15 +238,15 @@ __device__ inline void adj2_fetch(Adj2Plane<CPL>& P, const Adj2Args& a, int x, i
     for (int c = 0; c < CPL; ++c) {
       const bool ph = zc.pml[c] && active;
       const int pi = pb + zc.pidx[c];
-      P.lu[0][c] = ph ? a.luin[pi] : 0.0f;
-      P.lu[1][c] = ph ? a.luin[a.P + pi] : 0.0f;
-      P.lp[0][c] = ph ? a.lpin[pi] : 0.0f;
-      P.lp[1][c] = ph ? a.lpin[a.P + pi] : 0.0f;
+      gcfloat* lu = to_global(a.luin);
+      gcfloat* lp = to_global(a.lpin);
+      P.lu[0][c] = ph ? lu[pi] : 0.0f;
+      P.lu[1][c] = ph ? lu[a.P + pi] : 0.0f;
+      P.lp[0][c] = ph ? lp[pi] : 0.0f;
+      P.lp[1][c] = ph ? lp[a.P + pi] : 0.0f;
     }
   }
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    P.sig[k] = active ? uniform_load(a.mask, (k * a.X + x) * a.Y + yc) : 0.0f;
+  adj2_sig(P.sig, a, x, yc, active);
 }
 
 // Row-exchange buffers (cell c of lane l at [c * 64 + l]).
@@ -243,7 +258,7 @@ struct Adj2Lds {
   float sH2[WY + 3][2][64 * CPL];     // level-2 lH x, z of plane x-1
 };
 
-template <int CPL, int WY, bool DG>
+template <int CPL, int WY, bool DG, bool RING = false>
 __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& L,
                                            Adj2Carry<CPL>& s, const Adj2Plane<CPL>& cur,
                                            Adj2Plane<CPL>& nxt, const AdjCoef<CPL>& zc, int x,
@@ -261,9 +276,10 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
   // This plane's D^n (level 1 at x) and G, D^(n-1) (level 2 at x-1), owned
   // rows only, used in phases B and C.  Without DG they are requested before
   // the next plane's inputs: waiting for them (in-order completion) then
-  // leaves the prefetch in flight.
+  // leaves the prefetch in flight.  RING: the caller filled cur with every
+  // input of the plane, and nothing is fetched here.
   float d1[3][CPL], g[3][CPL], d2[3][CPL];
-  if (DG) {
+  if (DG || RING) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -275,7 +291,7 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
   } else {
     adj2_load_dg<CPL>(d1, g, d2, a, x, yc, zb, own && act, own && row_valid && l2);
   }
-  if (x + 1 <= xb)
+  if (!RING && x + 1 <= xb)
     adj2_fetch<CPL, DG>(nxt, a, x + 1, yc, w == NW - 1, zb, row_valid && x + 1 < a.X, zc, own,
                         row_valid, xa);
 
@@ -394,23 +410,25 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
       store_col<CPL, true>(a.G + (k * N + o), gn, zb, Z);
     }
     const int pb = ((x - 1) * Y + y) * a.NP;
+    gfloat* lu = to_global(a.luout);
+    gfloat* lp = to_global(a.lpout);
     if (CPL == 2 && a.pair) {
       if (zc.pml[0]) {
         const int pi = pb + zc.pidx[0];
-        *reinterpret_cast<float2*>(a.luout + pi) = make_float2(lu2[0][0], lu2[0][CPL - 1]);
-        *reinterpret_cast<float2*>(a.luout + a.P + pi) = make_float2(lu2[1][0], lu2[1][CPL - 1]);
-        *reinterpret_cast<float2*>(a.lpout + pi) = make_float2(lp2[0][0], lp2[0][CPL - 1]);
-        *reinterpret_cast<float2*>(a.lpout + a.P + pi) = make_float2(lp2[1][0], lp2[1][CPL - 1]);
+        *reinterpret_cast<gf32x2*>(lu + pi) = f32x2{lu2[0][0], lu2[0][CPL - 1]};
+        *reinterpret_cast<gf32x2*>(lu + a.P + pi) = f32x2{lu2[1][0], lu2[1][CPL - 1]};
+        *reinterpret_cast<gf32x2*>(lp + pi) = f32x2{lp2[0][0], lp2[0][CPL - 1]};
+        *reinterpret_cast<gf32x2*>(lp + a.P + pi) = f32x2{lp2[1][0], lp2[1][CPL - 1]};
       }
     } else {
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
         if (zc.pml[c]) {
           const int pi = pb + zc.pidx[c];
-          a.luout[pi] = lu2[0][c];
-          a.luout[a.P + pi] = lu2[1][c];
-          a.lpout[pi] = lp2[0][c];
-          a.lpout[a.P + pi] = lp2[1][c];
+          lu[pi] = lu2[0][c];
+          lu[a.P + pi] = lu2[1][c];
+          lp[pi] = lp2[0][c];
+          lp[a.P + pi] = lp2[1][c];
         }
       }
     }
@@ -439,6 +457,31 @@ __device__ __forceinline__ void adj2_plane(const Adj2Args& a, Adj2Lds<CPL, WY>& 
   for (int k = 0; k < 3; ++k) s.sigm[k] = cur.sig[k];
 }
 
+// Per-z coefficients of a lane's cells zb .. zb + CPL - 1 and their index in
+// the psi arrays.
+template <int CPL>
+__device__ inline AdjCoef<CPL> make_adjcoef(const Adj2Args& a, int zb) {
+  const int Z = a.Z;
+  gcfloat* zt = to_global(a.zt);
+  AdjCoef<CPL> zc;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const int z = zb + c;
+    const int zz = min(z, Z - 1);
+    zc.ikE[c] = zt[IKE * Z + zz];
+    zc.ikEp[c] = z + 1 < Z ? zt[IKE * Z + z + 1] : 0.0f;
+    zc.bE[c] = zt[BE * Z + zz];
+    zc.cE[c] = zt[CE * Z + zz];
+    zc.ikH[c] = zt[IKH * Z + zz];
+    zc.ikHm[c] = z >= 1 && z - 1 < Z ? zt[IKH * Z + z - 1] : 0.0f;
+    zc.bH[c] = zt[BH * Z + zz];
+    zc.cH[c] = zt[CH * Z + zz];
+    zc.pml[c] = z < Z && (z < a.lo || z >= Z - a.hi);
+    zc.pidx[c] = z < a.lo ? z : z - (Z - a.hi) + a.lo;
+  }
+  return zc;
+}
+
 // One tile (WY rows x LX planes) of one pair of reversed steps.
 template <int CPL, int WY, bool DG>
 __device__ __forceinline__ void adj2_tile(const Adj2Args& a, int tile, Adj2Lds<CPL, WY>& L) {
@@ -454,22 +497,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args& a, int tile, Adj2Lds<C
   const int xa = tx * a.LX;
   const int xb = min(xa + a.LX, a.X);
 
-  AdjCoef<CPL> zc;
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const int z = zb + c;
-    const int zz = min(z, Z - 1);
-    zc.ikE[c] = a.zt[IKE * Z + zz];
-    zc.ikEp[c] = z + 1 < Z ? a.zt[IKE * Z + z + 1] : 0.0f;
-    zc.bE[c] = a.zt[BE * Z + zz];
-    zc.cE[c] = a.zt[CE * Z + zz];
-    zc.ikH[c] = a.zt[IKH * Z + zz];
-    zc.ikHm[c] = z >= 1 && z - 1 < Z ? a.zt[IKH * Z + z - 1] : 0.0f;
-    zc.bH[c] = a.zt[BH * Z + zz];
-    zc.cH[c] = a.zt[CH * Z + zz];
-    zc.pml[c] = z < Z && (z < a.lo || z >= Z - a.hi);
-    zc.pidx[c] = z < a.lo ? z : z - (Z - a.hi) + a.lo;
-  }
+  const AdjCoef<CPL> zc = make_adjcoef<CPL>(a, zb);
 
   Adj2Carry<CPL> s;
 #pragma unroll
@@ -550,6 +578,271 @@ __global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2_batch(
   }
 }
 
+// ---------------------------------------------------------------- LDS ring
+// Two cells per lane with every input of a plane copied straight into LDS
+// two planes ahead (global_load_lds), as the forward's k_stream2_ring does:
+// lE and cb at x+1 and lH at x for every row, D^n at x and G, D^(n-1) at x-1
+// for the owned rows, the psi cotangents, and the top row's lE, cb of the
+// row above.  The copies take no registers (no next-plane struct, no D/G
+// prefetch), and two planes are in flight per wave instead of one.
+//
+// LDS: the row exchange (42 KB), the ring 2 x 7 x 9 columns (63 KB), the
+// owned rows' D/G ring 2 x 4 x 9 columns (36 KB), psi 7 KB, top row 4 KB:
+// 152 KB, one workgroup per CU as before.
+//
+// The copies are invisible to the compiler, so before reading plane x's
+// slot a wave waits until at most K vector memory operations are
+// outstanding: the fewest it surely issued after plane x's copies.  Those
+// are plane x+1's copies (7 a row; 9 on the top row; 12 on owned rows), plus
+// on owned rows 9 stores (lE, lH, G) of each of planes x-2 and x-1 that
+// stored (x-1 >= xa + 1; the psi stores are skipped where no lane is PML).
+// Nothing else the loop issues is a vector load.
+constexpr int kAdjRingPsi = 32;  // psi entries per row the ring holds (lo + hi)
+
+template <int WY>
+struct Adj2RingLds {
+  static constexpr int NW = WY + 3;
+  Adj2Lds<2, WY> xch;          // row exchange, as in k_adj_stream2
+  float ring[2][NW][9][128];   // lE x y z and cb x y z at x+1, lH x y z at x
+  float dring[2][WY][9][128];  // owned rows: D^n at x, G and D^(n-1) at x-1
+  float psr[2][NW][2][64];     // psi_H cotangent x|y, psi_E cotangent x|y
+  float topr[2][4][128];       // top row: lE x, z and cb x, z of the row above
+};
+
+// Requests plane p's inputs into ring slot `slot` (p <= X): lH, D^n, psi
+// and the top row at p, lE and cb at p+1, G and D^(n-1) at p-1, each plane
+// clamped into the domain (the reader zeroes what lies outside; at p = X the
+// level-2 data of plane X-1 is still needed).  16-byte copies carry two
+// columns: lanes 0-31 the even one, lanes 32-63 the odd one (4 cells a lane;
+// heights below 128 repeat the last).
+template <int WY>
+__device__ inline void adj2_ring_copy(Adj2RingLds<WY>& L, const Adj2Args& a, int slot, int w,
+                                      int p, int yc, bool top, bool own, int lane) {
+  const int N = a.N, Z = a.Z;
+  const int pl = min(p, a.X - 1);
+  const int oh = (pl * a.Y + yc) * Z;
+  const int oe = (min(p + 1, a.X - 1) * a.Y + yc) * Z;
+  const int half = lane >> 5;
+  const int dz = min((lane & 31) * 4, Z - 4);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = 2 * j + half;
+    const float* g = k < 3   ? a.lEin + (k * N + oe)
+                     : k < 6 ? a.cb + ((k - 3) * N + oe)
+                             : a.lHin + ((k - 6) * N + oh);
+    dma16(g + dz, lds_addr(&L.ring[slot][w][2 * j][0]));
+  }
+  if (lane < 32) dma16(a.lHin + (2 * N + oh) + dz, lds_addr(&L.ring[slot][w][8][0]));
+  const int e = lane & 31;
+  const int pb = (pl * a.Y + yc) * a.NP;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const float* base = q == 0 ? a.luin : a.lpin;
+    const float* g = a.NP > 0 ? base + (half * a.P + pb + min(e, a.NP - 1)) : a.lEin;
+    dma4(g, lds_addr(&L.psr[slot][w][q][0]));
+  }
+  if (top) {
+    const int ou = (pl * a.Y + min(yc + 1, a.Y - 1)) * Z + (half ? 2 * N : 0);
+    dma16(a.lEin + ou + dz, lds_addr(&L.topr[slot][0][0]));
+    dma16(a.cb + ou + dz, lds_addr(&L.topr[slot][2][0]));
+  }
+  if (own) {
+    const int om = (min(max(p - 1, 0), a.X - 1) * a.Y + yc) * Z;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 2 * j + half;
+      const float* g = k < 3   ? a.D1 + (k * N + oh)
+                       : k < 6 ? a.G + ((k - 3) * N + om)
+                               : a.D2 + ((k - 6) * N + om);
+      dma16(g + dz, lds_addr(&L.dring[slot][w - 2][2 * j][0]));
+    }
+    if (lane < 32) dma16(a.D2 + (2 * N + om) + dz, lds_addr(&L.dring[slot][w - 2][8][0]));
+  }
+}
+
+// Plane x's inputs from ring slot `slot`, zeroed where the register kernel's
+// fetch would not load them (absorber values come separately).
+template <int WY>
+__device__ inline void adj2_ring_read(Adj2Plane<2>& P, const Adj2RingLds<WY>& L,
+                                      const Adj2Args& a, int slot, int w, int x, int yc, bool top,
+                                      bool own, bool row_valid, int xa, int zb,
+                                      const AdjCoef<2>& zc) {
+  const bool active = row_valid && x < a.X;
+  const bool nx = active && x + 1 < a.X;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const f32x2 te = *reinterpret_cast<const f32x2*>(&L.ring[slot][w][k][zb]);
+    const f32x2 tc = *reinterpret_cast<const f32x2*>(&L.ring[slot][w][3 + k][zb]);
+    const f32x2 th = *reinterpret_cast<const f32x2*>(&L.ring[slot][w][6 + k][zb]);
+    P.le[k][0] = nx ? te.x : 0.0f;
+    P.le[k][1] = nx ? te.y : 0.0f;
+    P.cb[k][0] = nx ? tc.x : 0.0f;
+    P.cb[k][1] = nx ? tc.y : 0.0f;
+    P.lh[k][0] = active ? th.x : 0.0f;
+    P.lh[k][1] = active ? th.y : 0.0f;
+  }
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const bool ok = zc.pml[c] && active;
+      const int i = m * 32 + (zc.pml[c] ? zc.pidx[c] : 0);
+      P.lu[m][c] = ok ? L.psr[slot][w][0][i] : 0.0f;
+      P.lp[m][c] = ok ? L.psr[slot][w][1][i] : 0.0f;
+    }
+  // Every wave reads (clamped rows) and selects: zeroing fields on one side
+  // of a branch lets the compiler merge the two sides' stores into one
+  // through a selected address, which puts the struct in scratch memory.
+  const bool v = top && active && yc + 1 < a.Y;
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const f32x2 te = *reinterpret_cast<const f32x2*>(&L.topr[slot][m][zb]);
+    const f32x2 tc = *reinterpret_cast<const f32x2*>(&L.topr[slot][2 + m][zb]);
+    P.kue[m][0] = v ? te.x : 0.0f;
+    P.kue[m][1] = v ? te.y : 0.0f;
+    P.kuc[m][0] = v ? tc.x : 0.0f;
+    P.kuc[m][1] = v ? tc.y : 0.0f;
+  }
+  const int wd = own ? w - 2 : 0;
+  const bool l1 = own && active;
+  const bool l2 = own && row_valid && x >= xa && x >= 1;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const f32x2 t1 = *reinterpret_cast<const f32x2*>(&L.dring[slot][wd][k][zb]);
+    const f32x2 tg = *reinterpret_cast<const f32x2*>(&L.dring[slot][wd][3 + k][zb]);
+    const f32x2 t2 = *reinterpret_cast<const f32x2*>(&L.dring[slot][wd][6 + k][zb]);
+    P.d1[k][0] = l1 ? t1.x : 0.0f;
+    P.d1[k][1] = l1 ? t1.y : 0.0f;
+    P.g[k][0] = l2 ? tg.x : 0.0f;
+    P.g[k][1] = l2 ? tg.y : 0.0f;
+    P.d2[k][0] = l2 ? t2.x : 0.0f;
+    P.d2[k][1] = l2 ? t2.y : 0.0f;
+  }
+}
+
+template <int WY>
+__device__ __forceinline__ void adj2_ring_tile(const Adj2Args& a, int tile, Adj2RingLds<WY>& L) {
+  constexpr int NW = WY + 3;
+  const int Z = a.Z;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int zb = lane * 2;
+  const int ty = tile % a.nty, tx = tile / a.nty;
+  const int y = ty * WY - 2 + w;
+  const bool row_valid = y >= 0 && y < a.Y;
+  const int yc = min(max(y, 0), a.Y - 1);
+  const int xa = tx * a.LX;
+  const int xb = min(xa + a.LX, a.X);
+  const bool top = w == NW - 1;
+  const bool own = w >= 2 && w <= WY + 1;
+  const AdjCoef<2> zc = make_adjcoef<2>(a, zb);
+
+  Adj2Carry<2> s;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s.e1m[k][c] = s.cbm[k][c] = s.h1m[k][c] = s.gpm[k][c] = 0.0f;
+    s.lu1m[0][c] = s.lu1m[1][c] = s.lp1m[0][c] = s.lp1m[1][c] = 0.0f;
+    s.h2p[0][c] = s.h2p[1][c] = 0.0f;
+  }
+  s.sigm[0] = s.sigm[1] = s.sigm[2] = 0.0f;
+  const int xs = max(xa - 2, 0);
+  // Planes past xb are never read: their copies repeat plane xb's (L2 hits).
+  adj2_ring_copy<WY>(L, a, 0, w, xs, yc, top, own, lane);
+  adj2_ring_copy<WY>(L, a, 1, w, min(xs + 1, xb), yc, top, own, lane);
+  {
+    const int o = (xs * a.Y + yc) * Z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      load_col<2, true>(s.le0[k], a.lEin + (k * a.N + o), zb, Z, row_valid);
+      load_col<2, true>(s.cb0[k], a.cb + (k * a.N + o), zb, Z, row_valid);
+    }
+  }
+  // lK x, z of the first plane for the row below.
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    L.xch.sK1[0][w][0][c * 64 + lane] = s.cb0[0][c] * s.le0[0][c];
+    L.xch.sK1[0][w][1][c * 64 + lane] = s.cb0[2][c] * s.le0[2][c];
+  }
+  lds_barrier();
+  Adj2Plane<2> cur, unused;
+  float nsig[3];
+  adj2_sig(cur.sig, a, xs, yc, row_valid);
+  int par = 0;
+  for (int x = xs; x <= xb; ++x) {
+    const int slot = (x - xs) & 1;
+    // Wait for plane x's copies (see above for the counts).
+    if (own) {
+      const int n = row_valid ? (x - 1 >= xa + 1) + (x - 2 >= xa + 1) : 0;
+      if (n == 2) asm volatile("s_waitcnt vmcnt(30)" ::: "memory");
+      else if (n == 1) asm volatile("s_waitcnt vmcnt(21)" ::: "memory");
+      else asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
+    } else if (top) {
+      asm volatile("s_waitcnt vmcnt(9)" ::: "memory");
+    } else {
+      asm volatile("s_waitcnt vmcnt(7)" ::: "memory");
+    }
+    adj2_ring_read<WY>(cur, L, a, slot, w, x, yc, top, own, row_valid, xa, zb, zc);
+    // This wave's reads of the slot are done before plane x+2 overwrites it.
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    adj2_ring_copy<WY>(L, a, slot, w, min(x + 2, xb), yc, top, own, lane);
+    adj2_sig(nsig, a, x + 1, yc, x + 1 <= xb && row_valid && x + 1 < a.X);
+    adj2_plane<2, WY, false, true>(a, L.xch, s, cur, unused, zc, x, xa, xb, w, y, yc, lane, zb,
+                                   row_valid, par);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cur.sig[k] = nsig[k];
+    par ^= 1;
+  }
+  // The last planes' copies (repeats, never read) land before the ring is
+  // reused or the workgroup ends.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <int WY>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2_ring(Adj2Args a) {
+  __shared__ Adj2RingLds<WY> L;
+  int tile = blockIdx.x;
+  if ((a.ntiles & 7) == 0) tile = (tile & 7) * (a.ntiles >> 3) + (tile >> 3);
+  adj2_ring_tile<WY>(a, tile, L);
+}
+
+// Batched form of k_adj_stream2_ring (the items and views of
+// k_adj_stream2_batch).
+template <int WY>
+__global__ __launch_bounds__(64 * (WY + 3)) void k_adj_stream2_ring_batch(
+    const Adj2Args* __restrict__ args, const int* __restrict__ tile_start,
+    const int* __restrict__ lstart, int nsims, int total, int* __restrict__ counter, int launch,
+    int step, int seg_begin) {
+  __shared__ Adj2RingLds<WY> L;
+  __shared__ int s_item, s_next;
+  if (threadIdx.x == 0) s_next = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = pop_item(counter, total, s_next);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (t >= total) break;
+    int lo = 0, hi = nsims - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    if (step >= args[2 * lo].nsteps) continue;
+    Adj2Args a = args[2 * lo + ((launch - lstart[lo]) & 1)];
+    const long long f3 = 3LL * a.N;
+    a.D1 += (long long)(step - seg_begin) * f3;
+    a.D2 = a.D1 - f3;
+    adj2_ring_tile<WY>(a, t - tile_start[lo], L);
+  }
+}
+
+// Whether the LDS-ring form takes this problem: heights in (64, 128] that
+// are multiples of 4, with at most 32 PML cells.
+static bool adjoint_stream2_ring_supported(const Geom& g) {
+  return g.Z > 64 && g.Z <= 128 && g.Z % 4 == 0 && g.NP <= kAdjRingPsi;
+}
+
 // Whether the two-step reverse kernel handles this problem.
 bool adjoint_stream2_supported(const Geom& g) {
   return (g.Z <= 64) || (g.Z <= 128 && g.Z % 2 == 0);
@@ -588,6 +881,11 @@ static int adj2_resident_single() {
 template <int CPL>
 static int adj2_resident_batch() {
   static const int n = adj2_resident(&k_adj_stream2_batch<CPL, kAdj2WY>);
+  return n;
+}
+
+static int adj2_resident_ring_batch() {
+  static const int n = adj2_resident(&k_adj_stream2_ring_batch<kAdj2WY>);
   return n;
 }
 
@@ -641,6 +939,8 @@ hipError_t launch_adjoint_stream2(const Geom& g, const pjz_fdtd_problem& p, cons
   constexpr int WY = kAdj2WY;
   if (g.Z <= 64)
     hipLaunchKernelGGL((k_adj_stream2<1, WY>), dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
+  else if (adjoint_stream2_ring_supported(g))
+    hipLaunchKernelGGL((k_adj_stream2_ring<WY>), dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
   else
     hipLaunchKernelGGL((k_adj_stream2<2, WY>), dim3(a.ntiles), dim3(64 * (WY + 3)), 0, s, a);
   return hipGetLastError();
@@ -660,9 +960,11 @@ size_t adjoint2_batch_meta_bytes(int n, int S) {
 hipError_t adjoint2_batch_prepare(const BatchItem* items, int n, int S, const int* lstart,
                                   char* meta, Adj2BatchPlan& plan, hipStream_t s) {
   std::vector<int> cls[2];
+  plan.ring1 = true;
   for (int i = 0; i < n; ++i) {
     if (!adjoint_stream2_supported(items[i].g)) return hipErrorInvalidValue;
     cls[items[i].g.Z <= 64 ? 0 : 1].push_back(i);
+    if (items[i].g.Z > 64) plan.ring1 = plan.ring1 && adjoint_stream2_ring_supported(items[i].g);
   }
   plan.n = n;
   plan.S = std::max(S, 1);
@@ -684,8 +986,9 @@ hipError_t adjoint2_batch_prepare(const BatchItem* items, int n, int S, const in
       xs.push_back(items[i].g.X);
       ntys.push_back((items[i].g.Y + kAdj2WY - 1) / kAdj2WY);
     }
-    const int slots =
-        plan.cus * (c == 0 ? adj2_resident_batch<1>() : adj2_resident_batch<2>());
+    const int slots = plan.cus * (c == 0 ? adj2_resident_batch<1>()
+                                  : plan.ring1 ? adj2_resident_ring_batch()
+                                               : adj2_resident_batch<2>());
     const int cLX = batch_chunk_x(xs.data(), ntys.data(), (int)xs.size(), slots, kAdj2BatchLX);
     int t = 0;
     for (size_t k = 0; k < cls[c].size(); ++k) {
@@ -726,8 +1029,9 @@ hipError_t launch_adjoint2_batch_step(const Adj2BatchPlan& plan, int launch, int
   for (int c = 0; c < 2; ++c) {
     const int nc = plan.count[c];
     if (nc == 0) continue;
-    const int grid = std::min(
-        plan.total[c], plan.cus * (c == 0 ? adj2_resident_batch<1>() : adj2_resident_batch<2>()));
+    const int grid = std::min(plan.total[c], plan.cus * (c == 0        ? adj2_resident_batch<1>()
+                                                         : plan.ring1 ? adj2_resident_ring_batch()
+                                                                      : adj2_resident_batch<2>()));
     const Adj2Args* a = dargs + 2 * plan.base[c];
     const int* ts = dts + plan.base[c] + c;
     const int* ls = dls + plan.base[c];
@@ -735,6 +1039,9 @@ hipError_t launch_adjoint2_batch_step(const Adj2BatchPlan& plan, int launch, int
     if (c == 0)
       hipLaunchKernelGGL((k_adj_stream2_batch<1, WY>), dim3(grid), dim3(64 * (WY + 3)), 0, s, a,
                          ts, ls, nc, plan.total[c], ctr, launch, step, seg_begin);
+    else if (plan.ring1)
+      hipLaunchKernelGGL((k_adj_stream2_ring_batch<WY>), dim3(grid), dim3(64 * (WY + 3)), 0, s,
+                         a, ts, ls, nc, plan.total[c], ctr, launch, step, seg_begin);
     else
       hipLaunchKernelGGL((k_adj_stream2_batch<2, WY>), dim3(grid), dim3(64 * (WY + 3)), 0, s, a,
                          ts, ls, nc, plan.total[c], ctr, launch, step, seg_begin);

@@ -50,22 +50,6 @@ struct RingLds {
   float upr[2][2][128];     // top row: input Ex, Ez of the row above
 };
 
-typedef const __attribute__((address_space(3))) void* lds_cptr;
-
-__device__ inline unsigned lds_addr(const void* p) { return (unsigned)(size_t)(lds_cptr)p; }
-
-// Copies 16 bytes a lane from global memory to LDS at lds + 16 * lane.
-__device__ inline void dma16(const float* g, unsigned lds) {
-  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds)
-               : "memory", "m0");
-}
-
-// Copies 4 bytes a lane from global memory to LDS at lds + 4 * lane.
-__device__ inline void dma4(const float* g, unsigned lds) {
-  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(lds)
-               : "memory", "m0");
-}
-
 // Copies issued per plane by every wave (the top row issues one more).
 constexpr int kRingCopies = 7;
 // Stores every plane issues whatever its rows (E, H, and psi as pairs).

@@ -71,16 +71,17 @@ struct ZCoef {
 template <int CPL>
 __device__ inline ZCoef<CPL> make_zcoef(const Stream2Args& a, int zb) {
   const int Z = a.Z;
+  gcfloat* zt = to_global(a.zt);
   ZCoef<CPL> zc;
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const int z = min(zb + c, Z - 1);
-    zc.ikE[c] = a.zt[IKE * Z + z];
-    zc.bE[c] = a.zt[BE * Z + z];
-    zc.cE[c] = a.zt[CE * Z + z];
-    zc.ikH[c] = a.zt[IKH * Z + z];
-    zc.bH[c] = a.zt[BH * Z + z];
-    zc.cH[c] = a.zt[CH * Z + z];
+    zc.ikE[c] = zt[IKE * Z + z];
+    zc.bE[c] = zt[BE * Z + z];
+    zc.cE[c] = zt[CE * Z + z];
+    zc.ikH[c] = zt[IKH * Z + z];
+    zc.bH[c] = zt[BH * Z + z];
+    zc.cH[c] = zt[CH * Z + z];
     zc.pml[c] = (zb + c < Z) && (zb + c < a.lo || zb + c >= Z - a.hi);
     zc.pidx[c] = zb + c < a.lo ? zb + c : zb + c - (Z - a.hi) + a.lo;
   }
@@ -94,30 +95,32 @@ __device__ inline ZCoef<CPL> make_zcoef(const Stream2Args& a, int zb) {
 template <int CPL>
 __device__ inline void add_source2(const Stream2Args& a, int step, int x, int y, int zb,
                                    float (&cx)[CPL], float (&cy)[CPL], float (&cz)[CPL]) {
+  gcfloat* sfield = to_global(a.sfield);
+  gcfloat* swave = to_global(a.swave);
   float ju[CPL], jv[CPL];  // J of (y, z), (x, z) or (x, y) for axis 0, 1, 2
 #pragma unroll
   for (int c = 0; c < CPL; ++c) ju[c] = jv[c] = 0.0f;
   if (a.saxis == 0) {
     if (x == a.spos || x == a.spos - 1) {
-      const float wv = a.swave[2 * step + (x == a.spos ? 0 : 1)];
+      const float wv = swave[2 * step + (x == a.spos ? 0 : 1)];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
         if (zb + c < a.Z) {
           const int so = y * a.Z + zb + c;
-          ju[c] = a.sfield[so] * wv;
-          jv[c] = a.sfield[a.Y * a.Z + so] * wv;
+          ju[c] = sfield[so] * wv;
+          jv[c] = sfield[a.Y * a.Z + so] * wv;
         }
       }
     }
   } else if (a.saxis == 1) {
     if (y == a.spos || y == a.spos - 1) {
-      const float wv = a.swave[2 * step + (y == a.spos ? 0 : 1)];
+      const float wv = swave[2 * step + (y == a.spos ? 0 : 1)];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
         if (zb + c < a.Z) {
           const int so = x * a.Z + zb + c;
-          ju[c] = a.sfield[so] * wv;
-          jv[c] = a.sfield[a.X * a.Z + so] * wv;
+          ju[c] = sfield[so] * wv;
+          jv[c] = sfield[a.X * a.Z + so] * wv;
         }
       }
     }
@@ -125,11 +128,11 @@ __device__ inline void add_source2(const Stream2Args& a, int step, int x, int y,
     const int c0 = a.spos - zb;
     if (c0 >= 0 && c0 < CPL) {
       const int xy = x * a.Y + y, XY = a.X * a.Y;
-      float jx = a.sfield[xy] * a.swave[2 * step];
-      float jy = a.sfield[XY + xy] * a.swave[2 * step];
+      float jx = sfield[xy] * swave[2 * step];
+      float jy = sfield[XY + xy] * swave[2 * step];
       if (a.squad > 1) {
-        jx += a.sfield[2 * XY + xy] * a.swave[2 * step + 1];
-        jy += a.sfield[3 * XY + xy] * a.swave[2 * step + 1];
+        jx += sfield[2 * XY + xy] * swave[2 * step + 1];
+        jy += sfield[3 * XY + xy] * swave[2 * step + 1];
       }
 #pragma unroll
       for (int k = 0; k < CPL; ++k) {
@@ -256,8 +259,8 @@ __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y
     const int pi = pb + zc.pidx[0];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      float2 t = make_float2(0.0f, 0.0f);
-      if (ph && (k < 2 || e_rows)) t = *reinterpret_cast<const float2*>(a.psiIn + (k * a.P + pi));
+      f32x2 t = {0.0f, 0.0f};
+      if (ph && (k < 2 || e_rows)) t = *reinterpret_cast<gcf32x2*>(to_global(a.psiIn) + (k * a.P + pi));
       P.ps[k][0] = t.x;
       P.ps[k][CPL - 1] = t.y;
     }
@@ -266,10 +269,11 @@ __device__ inline void fetch2(Plane2<CPL>& P, const Stream2Args& a, int x, int y
     for (int c = 0; c < CPL; ++c) {
       const bool ph = zc.pml[c] && active;
       const int pi = pb + zc.pidx[c];
-      P.ps[0][c] = ph ? a.psiIn[pi] : 0.0f;
-      P.ps[1][c] = ph ? a.psiIn[a.P + pi] : 0.0f;
-      P.ps[2][c] = ph && e_rows ? a.psiIn[2 * a.P + pi] : 0.0f;
-      P.ps[3][c] = ph && e_rows ? a.psiIn[3 * a.P + pi] : 0.0f;
+      gcfloat* ps = to_global(a.psiIn);
+      P.ps[0][c] = ph ? ps[pi] : 0.0f;
+      P.ps[1][c] = ph ? ps[a.P + pi] : 0.0f;
+      P.ps[2][c] = ph && e_rows ? ps[2 * a.P + pi] : 0.0f;
+      P.ps[3][c] = ph && e_rows ? ps[3 * a.P + pi] : 0.0f;
     }
   }
   // One row per wave: the absorber value is wave-uniform (scalar load).

@@ -9,23 +9,41 @@ namespace pjz {
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Global (address space 1) views of device pointers.  Accesses through them
+// compile to global loads and stores, not flat ones.  A kernel whose
+// pointers come from an argument block in memory (the batched kernels) would
+// otherwise access memory with flat instructions, which count in lgkmcnt as
+// well as vmcnt: every LDS barrier (lgkmcnt(0)) would then wait for the
+// prefetched loads and the stores in flight.  A pending flat operation also
+// makes the compiler wait for every outstanding vector memory operation
+// (vmcnt(0)) at the next join.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const float gcfloat;
+typedef __attribute__((address_space(1))) f32x2 gf32x2;
+typedef __attribute__((address_space(1))) const f32x2 gcf32x2;
+typedef __attribute__((address_space(1))) f32x4 gf32x4;
+typedef __attribute__((address_space(1))) const f32x4 gcf32x4;
+__device__ inline gfloat* to_global(float* p) { return (gfloat*)p; }
+__device__ inline gcfloat* to_global(const float* p) { return (gcfloat*)p; }
+
 // Loads CPL consecutive z cells of a row; cells at z >= Z read as zero.  NT
 // marks the load nontemporal (streaming: the line is not kept for reuse).
 template <int CPL, bool vec, bool NT = false>
-__device__ inline void load_col(float (&v)[CPL], const float* __restrict__ p, int zb, int Z,
+__device__ inline void load_col(float (&v)[CPL], const float* __restrict__ pg, int zb, int Z,
                                 bool valid) {
   if (!valid || zb >= Z) {
 #pragma unroll
     for (int c = 0; c < CPL; ++c) v[c] = 0.0f;
     return;
   }
+  gcfloat* p = to_global(pg);
   if (vec) {
     if constexpr (CPL == 4) {
-      const f32x4* q = reinterpret_cast<const f32x4*>(p + zb);
+      gcf32x4* q = reinterpret_cast<gcf32x4*>(p + zb);
       const f32x4 t = NT ? __builtin_nontemporal_load(q) : *q;
       v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
     } else if constexpr (CPL == 2) {
-      const f32x2* q = reinterpret_cast<const f32x2*>(p + zb);
+      gcf32x2* q = reinterpret_cast<gcf32x2*>(p + zb);
       const f32x2 t = NT ? __builtin_nontemporal_load(q) : *q;
       v[0] = t.x; v[1] = t.y;
     } else if constexpr (CPL == 3) {
@@ -46,15 +64,16 @@ __device__ inline void load_col(float (&v)[CPL], const float* __restrict__ p, in
 // them out of the cache leaves room for the inputs that neighbouring tiles
 // share (ghost rows).
 template <int CPL, bool vec>
-__device__ inline void store_col(float* __restrict__ p, const float (&v)[CPL], int zb, int Z) {
+__device__ inline void store_col(float* __restrict__ pg, const float (&v)[CPL], int zb, int Z) {
   if (zb >= Z) return;
+  gfloat* p = to_global(pg);
   if (vec) {
     if constexpr (CPL == 4) {
       f32x4 t = {v[0], v[1], v[2], v[3]};
-      __builtin_nontemporal_store(t, reinterpret_cast<f32x4*>(p + zb));
+      __builtin_nontemporal_store(t, reinterpret_cast<gf32x4*>(p + zb));
     } else if constexpr (CPL == 2) {
       f32x2 t = {v[0], v[1]};
-      __builtin_nontemporal_store(t, reinterpret_cast<f32x2*>(p + zb));
+      __builtin_nontemporal_store(t, reinterpret_cast<gf32x2*>(p + zb));
     } else if constexpr (CPL == 3) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) __builtin_nontemporal_store(v[c], p + zb + c);
@@ -67,16 +86,6 @@ __device__ inline void store_col(float* __restrict__ p, const float (&v)[CPL], i
       if (zb + c < Z) p[zb + c] = v[c];
   }
 }
-
-// Global (address space 1) views of generic device pointers.  Stores through
-// them compile to global_store, not flat_store: a pending flat operation makes
-// the compiler wait for every outstanding vector memory operation (vmcnt(0))
-// at the next join, which at a loop's end also waits for the stores just
-// issued, not only for the prefetched loads the next iteration needs.
-typedef __attribute__((address_space(1))) float gfloat;
-typedef __attribute__((address_space(1))) f32x2 gf32x2;
-typedef __attribute__((address_space(1))) f32x4 gf32x4;
-__device__ inline gfloat* to_global(float* p) { return (gfloat*)p; }
 
 // store_col through a global pointer.
 template <int CPL, bool vec>
@@ -208,6 +217,27 @@ __device__ inline void z_prev(float (&o)[CPL], const float (&v)[CPL], int lane) 
   o[0] = lane == 0 ? 0.0f : dn;
 #pragma unroll
   for (int c = 1; c < CPL; ++c) o[c] = v[c - 1];
+}
+
+// Copies from global memory straight into LDS (gfx950 global_load_lds):
+// the LDS address goes in M0 and is wave-uniform; lane i writes its bytes
+// at lds + i * size.  Issued by inline asm, so the compiler neither counts
+// nor waits for them: a kernel using them waits itself (s_waitcnt vmcnt)
+// before reading what they wrote.
+typedef const __attribute__((address_space(3))) void* lds_cptr;
+
+__device__ inline unsigned lds_addr(const void* p) { return (unsigned)(size_t)(lds_cptr)p; }
+
+// 16 bytes a lane.
+__device__ inline void dma16(const float* g, unsigned lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds)
+               : "memory", "m0");
+}
+
+// 4 bytes a lane.
+__device__ inline void dma4(const float* g, unsigned lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(lds)
+               : "memory", "m0");
 }
 
 // Workgroup barrier that waits for this wave's LDS (and scalar) operations

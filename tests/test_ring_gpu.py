@@ -5,6 +5,9 @@ of 4, at most 32 PML cells; the default two-step kernel on those shapes with
 register-prefetch two-step kernel with 12-row tiles (<= 1e-6: the same
 per-cell arithmetic, another tiling).
 
+The gradient tests also cover the reverse kernel's LDS-ring form
+(k_adj_stream2_ring, the same shapes).
+
 Covered: every source axis; heights 68, 100, 116 and 128; PML widths odd
 (psi stored cell by cell), even (psi in pairs), summing to 32 (the ring's
 limit) and zero; x chunks of 1-3 planes (ghost planes before the chunk start
@@ -75,9 +78,14 @@ def test_ring_z_as_batch():
   assert T.rel_l2(run(kw), C.fdtdz(**kw)) <= T.TOL
 
 
-@pytest.mark.parametrize("Z,widths", [(128, (10, 10)), (100, (6, 5))])
-def test_ring_gradient_matches_adjoint_oracle(Z, widths):
-  kw = T.random_problem(0, X=20, Y=17, Z=Z, tt=60, seed=Z, widths=widths)
+@pytest.mark.parametrize("axis", [0, 2])
+@pytest.mark.parametrize("Z,widths", [(128, (10, 10)), (100, (6, 5)), (116, (16, 16)),
+                                      (68, (0, 0)), (100, (3, 8))])
+def test_ring_gradient_matches_adjoint_oracle(Z, widths, axis):
+  """The forward's segment recomputes and the reverse pairs both run the
+  LDS-ring kernels here (k_stream2_ring with captures, k_adj_stream2_ring)."""
+  print(f"case Z={Z} widths={widths} axis={axis}", flush=True)
+  kw = T.random_problem(axis, X=20, Y=17, Z=Z, tt=60, seed=Z + axis, widths=widths)
   eps = kw.pop("epsilon")
   rng = np.random.default_rng(3)
   nn = len(range(*kw["output_steps"]))
@@ -102,3 +110,4 @@ def test_ring_native_shape_short_run():
   print(f"native 512x512x128 ring vs 12-row kernel: {err:.2e}", flush=True)
   assert err <= 1e-6
   assert T.rel_l2(got, C.fdtdz(**kw)) <= T.TOL
+
