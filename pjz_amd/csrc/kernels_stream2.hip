@@ -365,13 +365,16 @@ bool stream2_supported(const Geom& g) {
           (g.Z % 4 == 0));
 }
 
-// Heights 64 < zz <= 96 that are multiples of 3 (pjz's default budget is
-// 128 - 32 = 96) run three cells per lane with two rows per wave (32 lanes
-// each): every lane holds cells, where two cells per lane would leave a
-// quarter of the lanes empty at 96.  tile_y 4, 8, 12 or 13 asks for the
-// two-cells-per-lane kernel instead (even heights).
+// Heights 64 < zz <= 96 that are multiples of 3 run three cells per lane
+// with two rows per wave (32 lanes each): every lane holds cells, where two
+// cells per lane leave some lanes empty (a quarter at 96).  Where the
+// LDS-ring kernel takes the height (multiples of 4, at most 32 PML cells;
+// pjz's default budget, 128 - 32 = 96, is one) it is faster even so: 107.6
+// against 100.7 Gcell-updates/s at 512x512x96.  tile_y 4, 8, 12 or 13 asks
+// for the two-cells-per-lane kernels (even heights).
 static bool use_cpl3(const Geom& g, const pjz_fdtd_problem& p) {
   if (g.Z <= 64 || g.Z > 96 || g.Z % 3 != 0) return false;
+  if (stream2_ring_supported(g)) return false;
   const bool cpl2_asked = p.tile_y == 4 || p.tile_y == 8 || p.tile_y == 12 || p.tile_y == 13;
   return !(cpl2_asked && g.Z % 2 == 0);
 }

@@ -1,6 +1,8 @@
-"""Heights 64 < zz <= 96 (multiples of 3): three cells per lane, two rows
-per wave.  pjz's default SimParams give zz = 128 - 32 = 96
+"""Heights 64 < zz <= 96.  pjz's default SimParams give zz = 128 - 32 = 96
 (src/pjz/_field.py:56-58), so every default field() call runs this path.
+Multiples of 4 with at most 32 PML cells (96 among them) run the LDS-ring
+kernel at two cells per lane; other multiples of 3 (69, 90, 93) run three
+cells per lane with two rows per wave.
 
 Checked against the FP64 oracle (relative L2 <= 1e-4) on random problems
 with ragged tiles (rows not a multiple of 13, fewer rows than a tile), every
@@ -26,6 +28,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("X,Y,Z,widths", [(24, 20, 96, (6, 5)), (19, 31, 96, (16, 16)),
+                                          (24, 20, 90, (6, 5)), (19, 31, 90, (16, 16)),
                                           (17, 9, 93, (4, 7)), (30, 14, 69, (0, 3))])
 def test_three_cells_per_lane_matches_oracle(axis, X, Y, Z, widths):
   print(f"case axis={axis} shape={X}x{Y}x{Z} pml={widths}", flush=True)
@@ -39,11 +42,12 @@ def test_three_cells_per_lane_matches_oracle(axis, X, Y, Z, widths):
     assert T.rel_l2(got, two) <= 1e-6
 
 
-def test_three_cells_per_lane_z_as_batch():
-  kw = T.random_problem(0, X=22, Y=27, Z=96, tt=70, seed=5, widths=(16, 16))
-  kw["pml_kappa"] = np.full((96, 2), np.inf, np.float32)
-  kw["pml_sigma"] = np.zeros((96, 2), np.float32)
-  kw["pml_alpha"] = np.zeros((96, 2), np.float32)
+@pytest.mark.parametrize("Z", [96, 90])
+def test_three_cells_per_lane_z_as_batch(Z):
+  kw = T.random_problem(0, X=22, Y=27, Z=Z, tt=70, seed=5, widths=(16, 16))
+  kw["pml_kappa"] = np.full((Z, 2), np.inf, np.float32)
+  kw["pml_sigma"] = np.zeros((Z, 2), np.float32)
+  kw["pml_alpha"] = np.zeros((Z, 2), np.float32)
   assert T.rel_l2(T.run_gpu(kw, "stream2"), C.fdtdz(**kw)) <= T.TOL
 
 
@@ -65,11 +69,12 @@ def test_field_with_default_simparams(gpu):
   assert err <= 1e-4
 
 
-def test_gradient_at_height_96():
-  """The gradient at zz = 96: checkpoint recomputes run the three-cells-per-
-  lane kernel (capturing E changes), the reverse steps run at two cells per
-  lane."""
-  kw = T.random_problem(0, X=20, Y=17, Z=96, tt=70, seed=21, widths=(6, 5))
+@pytest.mark.parametrize("Z", [96, 90])
+def test_gradient_at_height_96(Z):
+  """The gradient at zz = 96 (the LDS-ring kernels forward and reverse) and
+  90 (checkpoint recomputes on the three-cells-per-lane kernel, capturing E
+  changes; the reverse steps at two cells per lane)."""
+  kw = T.random_problem(0, X=20, Y=17, Z=Z, tt=70, seed=21, widths=(6, 5))
   eps = kw.pop("epsilon")
   rng = np.random.default_rng(3)
   nn = len(range(*kw["output_steps"]))
@@ -80,5 +85,5 @@ def test_gradient_at_height_96():
   (out * torch.as_tensor(g, device="cuda")).sum().backward()
   got = e.grad.cpu().numpy()
   err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
-  print(f"gradient zz=96: rel L2 {err:.2e}", flush=True)
+  print(f"gradient zz={Z}: rel L2 {err:.2e}", flush=True)
   assert err <= 1e-3
