@@ -630,7 +630,9 @@ __device__ inline void adj2_ring_copy(Adj2RingLds<WY>& L, const Adj2Args& a, int
     const float* g = k < 3   ? a.lEin + (k * N + oe)
                      : k < 6 ? a.cb + ((k - 3) * N + oe)
                              : a.lHin + ((k - 6) * N + oh);
-    dma16(g + dz, lds_addr(&L.ring[slot][w][2 * j][0]));
+    // cb x+1 (columns 4 and 5) nontemporal, as D below (read once)
+    if (j == 2) dma16nt(g + dz, lds_addr(&L.ring[slot][w][2 * j][0]));
+    else dma16(g + dz, lds_addr(&L.ring[slot][w][2 * j][0]));
   }
   if (lane < 32) dma16(a.lHin + (2 * N + oh) + dz, lds_addr(&L.ring[slot][w][8][0]));
   const int e = lane & 31;
@@ -654,9 +656,11 @@ __device__ inline void adj2_ring_copy(Adj2RingLds<WY>& L, const Adj2Args& a, int
       const float* g = k < 3   ? a.D1 + (k * N + oh)
                        : k < 6 ? a.G + ((k - 3) * N + om)
                                : a.D2 + ((k - 6) * N + om);
-      dma16(g + dz, lds_addr(&L.dring[slot][w - 2][2 * j][0]));
+      // copies of D^n or D^(n-1) alone nontemporal (grad_256: -0.6%)
+      if (j == 0 || j == 3) dma16nt(g + dz, lds_addr(&L.dring[slot][w - 2][2 * j][0]));
+      else dma16(g + dz, lds_addr(&L.dring[slot][w - 2][2 * j][0]));
     }
-    if (lane < 32) dma16(a.D2 + (2 * N + om) + dz, lds_addr(&L.dring[slot][w - 2][8][0]));
+    if (lane < 32) dma16nt(a.D2 + (2 * N + om) + dz, lds_addr(&L.dring[slot][w - 2][8][0]));
   }
 }
 

@@ -74,9 +74,11 @@ __device__ inline void ring_copy(RingLds<WY>& L, const Stream2Args& a, int slot,
     const float* g = k < 3   ? a.Ein + (k * N + oe)
                      : k < 6 ? a.Ein + (a.hoff + (k - 3) * N + oh)
                              : a.cb + ((k - 6) * N + oh);
-    dma16(g + dz, lds_addr(&L.ring[slot][w][2 * j][0]));
+    // cb is read once per plane and row: nontemporal (0.3% faster)
+    if (j == 3) dma16nt(g + dz, lds_addr(&L.ring[slot][w][2 * j][0]));
+    else dma16(g + dz, lds_addr(&L.ring[slot][w][2 * j][0]));
   }
-  if (lane < 32) dma16(a.cb + (2 * N + oh) + dz, lds_addr(&L.ring[slot][w][8][0]));
+  if (lane < 32) dma16nt(a.cb + (2 * N + oh) + dz, lds_addr(&L.ring[slot][w][8][0]));
   // psi: lanes 0-31 component 2q, lanes 32-63 component 2q + 1
   const int e = lane & 31;
   const int pb = (p * a.Y + yc) * a.NP;

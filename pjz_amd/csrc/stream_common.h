@@ -234,6 +234,12 @@ __device__ inline void dma16(const float* g, unsigned lds) {
                : "memory", "m0");
 }
 
+// 16 bytes a lane, nontemporal (data read once).
+__device__ inline void dma16nt(const float* g, unsigned lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off nt" ::"v"(g), "s"(lds)
+               : "memory", "m0");
+}
+
 // 4 bytes a lane.
 __device__ inline void dma4(const float* g, unsigned lds) {
   asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(lds)
