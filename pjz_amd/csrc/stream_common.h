@@ -196,11 +196,26 @@ __device__ inline void bstore(const float (&v)[CPL], rsrc_t r, unsigned voff, un
 typedef const __attribute__((address_space(4))) float* const_fptr;
 __device__ inline float uniform_load(const float* p, int i) { return ((const_fptr)p)[i]; }
 
+// Lane l+1's value (lane_next) or lane l-1's (lane_prev) across the whole
+// wave, 0 past its ends: DPP moves (wave_shl:1, wave_shr:1), plain VALU
+// instructions.  A shuffle would go through the LDS crossbar (ds_bpermute),
+// which competes with the kernels' LDS traffic and has to be waited for.
+__device__ inline float lane_next(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+}
+__device__ inline float lane_prev(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+}
+
 // Cell z+1 for each of the lane's cells (0 at and beyond Z).  A z column
 // spans W lanes (64, or 32 when a wave holds two rows).
 template <int CPL, int W = 64>
 __device__ inline void z_next(float (&o)[CPL], const float (&v)[CPL], int zb, int Z) {
-  float up = __shfl_down(v[0], 1, W);
+  float up;
+  if constexpr (W == 64) up = lane_next(v[0]);
+  else up = __shfl_down(v[0], 1, W);
 #pragma unroll
   for (int c = 0; c < CPL - 1; ++c) o[c] = v[c + 1];
   o[CPL - 1] = up;
@@ -213,7 +228,9 @@ __device__ inline void z_next(float (&o)[CPL], const float (&v)[CPL], int zb, in
 // within the column's W lanes.
 template <int CPL, int W = 64>
 __device__ inline void z_prev(float (&o)[CPL], const float (&v)[CPL], int lane) {
-  float dn = __shfl_up(v[CPL - 1], 1, W);
+  float dn;
+  if constexpr (W == 64) dn = lane_prev(v[CPL - 1]);
+  else dn = __shfl_up(v[CPL - 1], 1, W);
   o[0] = lane == 0 ? 0.0f : dn;
 #pragma unroll
   for (int c = 1; c < CPL; ++c) o[c] = v[c - 1];
