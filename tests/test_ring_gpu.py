@@ -13,7 +13,8 @@ Covered: every source axis; heights 68, 100, 116 and 128; PML widths odd
 limit) and zero; x chunks of 1-3 planes (ghost planes before the chunk start
 and ring copies clamped at the chunk end); step counts leaving a one-step
 tail; snapshots after every step; z as batch; the gradient (segment
-recomputes capture every level's change); and 512x512x128 for 24 steps.
+recomputes capture every level's change); domains one or two planes or rows
+thin, forward and gradient; and 512x512x128 for 24 steps.
 """
 
 import numpy as np
@@ -23,6 +24,7 @@ import torch
 import test_fdtdz_gpu as T
 from oracle import adjoint
 from oracle import c_oracle as C
+from oracle import helpers as H
 from pjz_amd import fdtdz_torch as F
 
 pytestmark = pytest.mark.gpu
@@ -111,3 +113,42 @@ def test_ring_native_shape_short_run():
   assert err <= 1e-6
   assert T.rel_l2(got, C.fdtdz(**kw)) <= T.TOL
 
+
+def thin_problem(X, Y, Z, widths=(6, 5), tt=40):
+  """A z-source problem on the whole (X, Y, Z) grid, any X, Y >= 1."""
+  rng = np.random.default_rng(X * Y + Z)
+  kw = T.random_problem(2, X=20, Y=20, Z=Z, tt=tt, seed=Z, widths=widths)
+  kw.update(epsilon=(1 + 4 * rng.random((3, X, Y, Z))).astype(np.float32),
+            source_field=rng.standard_normal((2, 2, X, Y, 1)).astype(np.float32),
+            source_position=Z // 2, offset=(0, 0, 0),
+            absorption_mask=(0.05 * rng.random((3, X, Y))).astype(np.float32),
+            output_steps=(tt - 10, tt, 3))
+  return kw
+
+
+@pytest.mark.parametrize("shape", [(1, 9, 128), (2, 3, 68), (5, 1, 100), (3, 2, 96),
+                                   (40, 1, 128)])
+def test_ring_thin_domains(shape):
+  """One or two planes or rows: ghost rows and planes past every edge, the
+  top row's copy of the row above clamped, ring copies repeating the last
+  plane from the first."""
+  print(f"case shape={shape}", flush=True)
+  kw = thin_problem(*shape)
+  assert T.rel_l2(run(kw), C.fdtdz(**kw)) <= T.TOL
+
+
+@pytest.mark.parametrize("shape", [(2, 3, 68), (5, 1, 100)])
+def test_ring_thin_domains_gradient(shape):
+  kw = thin_problem(*shape, tt=30)
+  eps = kw.pop("epsilon")
+  rng = np.random.default_rng(4)
+  nn = len(range(*kw["output_steps"]))
+  g = rng.standard_normal((nn, 3) + eps.shape[1:]).astype(np.float32)
+  ref = adjoint.grad_epsilon(eps, g_out=g, **kw)
+  e = torch.tensor(eps, device="cuda", requires_grad=True)
+  out = F.fdtdz(e, **dict(kw, launch_params=dict(kernel="stream2", ckpt_every=8)))
+  (out * torch.as_tensor(g, device="cuda")).sum().backward()
+  got = e.grad.cpu().numpy()
+  err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+  print(f"gradient {shape}: rel L2 {err:.2e}", flush=True)
+  assert err <= 1e-3
