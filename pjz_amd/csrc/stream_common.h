@@ -240,7 +240,10 @@ __device__ inline void z_prev(float (&o)[CPL], const float (&v)[CPL], int lane) 
 // the LDS address goes in M0 and is wave-uniform; lane i writes its bytes
 // at lds + i * size.  Issued by inline asm, so the compiler neither counts
 // nor waits for them: a kernel using them waits itself (s_waitcnt vmcnt)
-// before reading what they wrote.
+// before reading what they wrote.  The compiler does not preserve M0 across
+// an asm statement that clobbers it, so a kernel using them must make no
+// other use of M0 (no indirect register indexing, messages or LDS copies of
+// the compiler's own); tests/test_lds_dma_isa.py checks the built code.
 typedef const __attribute__((address_space(3))) void* lds_cptr;
 
 __device__ inline unsigned lds_addr(const void* p) { return (unsigned)(size_t)(lds_cptr)p; }
