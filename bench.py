@@ -14,7 +14,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                        [--no-cpu-baseline]
 
 --workload native_512_z64 runs the same at 512x512x64, where the default
-kernel takes three steps per launch.
+kernel takes three steps per launch; --workload native_512_z96 at
+512x512x96, pjz's default height.
 
 Other workloads (one JSON line each): --workload grad_256 (forward +
 adjoint gradient, 256x256x128, 6000 steps), --workload batch64 (64 uneven

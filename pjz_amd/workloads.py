@@ -57,6 +57,12 @@ def native_512_z64(steps=10000, seed=0):
   return native_512(steps=steps, seed=seed, zz=64)
 
 
+def native_512_z96(steps=10000, seed=0):
+  """native_512 at 512x512x96, the height pjz's default SimParams give
+  (zz = 128 - 32)."""
+  return native_512(steps=steps, seed=seed, zz=96)
+
+
 def native_512(steps=10000, seed=0, xx=512, yy=512, zz=128, design=200):
   """The 512x512x128 workload described above; ``xx``, ``yy``, ``zz`` and
   the ``design`` square's side shrink it for parity runs the oracle can
